@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Headline benchmark: SpMM GFLOP/s and HBM-roofline fraction on one MI355X.
+
+Workload (one "step" = one C = A*B over the whole matrix, inputs resident in
+HBM): the power-law matrix M=K=262144, row lengths Zipf(1.8) clipped to
+[1, 5000] (nnz = 3,191,107), values N(0,1), dense B 262144 x 128 FP32
+row-major in U[-1,1).  The native path runs with the variant and tile the
+per-matrix autotuner picks.  Also reported on the same matrix: SpMV (N=1),
+SpMM N=32, and the reference's eight algorithms at N=128 through the
+reference-compatible dispatcher.
+
+Prints ONE JSON line on stdout (everything else goes to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "da-spmmul-kernel_amd")]
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def traffic_from_profile(kernel_prefix, workload_tag):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the
+    copy-kernel calibration factor recorded in the same file); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload_tag}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        k = d["kernels"][kernel_prefix]
+        return int(round((k["FETCH_SIZE_KiB"] + k["WRITE_SIZE_KiB"]) * 1024 * d["calibration"]))
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def run_native(D, m, d_B, d_C, nv, warmup, steps, tune_reps=5):
+    best, _ = m.autotune(d_B, d_C, nv, reps=tune_reps)
+    D.synchronize()
+    t0 = time.perf_counter()
+    mean, per, main = m.benchmark(d_B, d_C, nv, warmup=warmup, reps=steps)
+    D.synchronize()
+    wall = time.perf_counter() - t0
+    return best, mean, per, main, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=128, help="dense columns of the headline SpMM")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU work of the cpu_baseline sample")
+    ap.add_argument("--no-extra", action="store_true", help="skip N=1/N=32/reference sweep")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: the SpMM path here runs on a single GPU and "
+              f"--gpus {args.gpus} is not supported (DESIGN.md describes how it would shard "
+              f"by rows).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import dspmm as D
+    import workloads as W
+
+    info = D.device_info()
+    log(f"device: {info['name']} {info['arch']} CUs={info['compute_units']}")
+    t = time.perf_counter()
+    a = W.c2_matrix()
+    nv = args.n
+    B = W.dense_b(2, a.nc, 128)
+    log(f"c2: M=K={a.nr} nnz={a.nnz} max row {int(np.diff(a.rowPtr).max())} "
+        f"(generated in {time.perf_counter() - t:.1f}s)")
+    m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.zeros((a.nr, 128))
+
+    best, mean, per, main_k, wall = run_native(D, m, d_B, d_C, nv, args.warmup, args.steps)
+    ms = mean
+    ms_main = float(np.mean(main_k))
+    fl = W.flops(a.nnz, nv)
+    gflops = fl / (ms * 1e-3) / 1e9
+    alg_bytes = W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv)
+    achieved = alg_bytes / (ms_main * 1e-3) / 1e9
+    log(f"native N={nv}: {best.as_dict()} mean {ms * 1e3:.1f} us/step, main kernel "
+        f"{ms_main * 1e3:.1f} us, median {np.median(per) * 1e3:.1f} us, wall {wall:.3f}s "
+        f"-> {gflops:.0f} GFLOP/s, {achieved:.0f} GB/s algorithmic")
+
+    extra = {"autotuned_config": best.as_dict(),
+             "ms_per_step_median": float(np.median(per)),
+             "ms_main_kernel_mean": ms_main,
+             "step_wall_s": wall,
+             "algorithmic_bytes_per_step": alg_bytes}
+    if not args.no_extra:
+        for n2 in (1, 32):
+            b2, mean2, per2, main2, _ = run_native(D, m, d_B, d_C, n2, args.warmup, args.steps)
+            by2 = W.algorithmic_bytes(a.nr, a.nc, a.nnz, n2)
+            extra[f"N{n2}"] = {"config": b2.as_dict(), "ms": mean2,
+                               "GFLOP/s": W.flops(a.nnz, n2) / (mean2 * 1e-3) / 1e9,
+                               "GB/s_algorithmic": by2 / (float(np.mean(main2)) * 1e-3) / 1e9,
+                               "roofline_frac": by2 / (float(np.mean(main2)) * 1e-3) / 1e9
+                               / PEAK_HBM_GBS}
+            log(f"native N={n2}: {extra[f'N{n2}']}")
+        # The reference's eight algorithms through the compat dispatcher
+        # (packed operands, ld = nv), named as the reference's CSV (run.sh:8).
+        d_rp = D.DeviceArray.from_numpy(a.rowPtr)
+        d_ri = D.DeviceArray.from_numpy(a.rowIdx())
+        d_ci = D.DeviceArray.from_numpy(a.colIdx)
+        d_v = D.DeviceArray.from_numpy(a.values)
+        prob = D.SpmvSpmmProblem(a.nr, a.nc, a.nnz, 128, d_rp, d_ri, d_ci, d_v, d_B, d_C)
+        sweep = {}
+        for layout in (D.DENSE_ROW_MAJOR, D.DENSE_COL_MAJOR):
+            for alg in (D.ALG_CSR_SCALAR, D.ALG_CSR_VECTOR, D.ALG_COO_SCALAR, D.ALG_COO_VECTOR):
+                t_ms = prob.benchmark(layout, nv, alg, 3, 20)
+                sweep[D.CSV_NAMES[(alg, layout)]] = fl / (t_ms * 1e-3) / 1e9
+        extra[f"reference_algorithms_N{nv}_GFLOP/s"] = sweep
+        log(f"reference algorithms N={nv}: {sweep}")
+
+    # CPU baseline: the restated reference host loop, one thread, bounded sample.
+    import importlib
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    O = importlib.import_module("oracle")
+    dense = np.ascontiguousarray(B.reshape(-1)[: a.nc * nv])   # same packing as the GPU run
+    t_one = O.time_host32(O.ROW_MAJOR, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, dense, reps=1)
+    reps = max(1, int(math.ceil(args.cpu_seconds / max(t_one, 1e-3))) - 1)
+    t_cpu = O.time_host32(O.ROW_MAJOR, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, dense,
+                          reps=reps)
+    cpu_value = fl / t_cpu / 1e9
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    cpu_baseline = {
+        "value": cpu_value, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+        "sample": (f"same matrix and N={nv}: {reps + 1} full SpMMs ({t_one + reps * t_cpu:.1f} s) "
+                   f"of oracle_spmm_host32 (restated cpuSpmvReference, FP32, -O2 "
+                   f"-ffp-contract=off) on 1 thread; host {cpu_model()}, "
+                   f"{os.cpu_count()} logical CPUs, affinity {affinity}"),
+    }
+    log(f"cpu baseline: {cpu_value:.2f} GFLOP/s ({t_cpu * 1e3:.0f} ms per SpMM)")
+
+    line = {
+        "metric": "SpMM GFLOP/s + % HBM roofline; batched req/s & p99 latency vs plain launches",
+        "value": gflops,
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator; reference ships no matrices)",
+        "config": {"workload": (f"power-law SpMM: M=K={a.nr}, Zipf(1.8) rows on [1,5000], "
+                                f"nnz={a.nnz}, N={nv}, FP32, row-major, C overwritten"),
+                   "M": a.nr, "K": a.nc, "nnz": a.nnz, "N": nv, "layout": "row-major",
+                   "variant": best.as_dict()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic_from_profile("k_nnz_tiled", f"c2_n{nv}")},
+        "cpu_baseline": cpu_baseline,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,613 @@
+// api_native.cpp -- C-ABI of the native path: device-resident CSR handle
+// (the upload), nnz-tile plans, dspmm_spmm_ex dispatch, the per-matrix
+// autotuner and the timed benchmark loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "internal.hpp"
+#include "native.hpp"
+
+namespace dspmm {
+namespace {
+
+struct NbPlan {
+    int S = 0;
+    int ntiles = 0, nsplits = 0, nslots = 0;
+    int4 *d_tiles = nullptr;
+    int4 *d_splits = nullptr;
+    size_t bytes = 0;
+};
+
+// Cuts rows [0, nr) into tiles of <= S nonzeros and <= S rows that start and
+// end on row boundaries; a row longer than S becomes ceil(len/S) chunks of
+// S nonzeros, each with its own carry slot, and one split record.
+void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &tiles,
+                 std::vector<int4> &splits, int &nslots) {
+    tiles.clear();
+    splits.clear();
+    nslots = 0;
+    int r = 0;
+    while (r < nr) {
+        const int len = rp[(size_t)r + 1] - rp[(size_t)r];
+        if (len > S) {
+            const int nch = (len + S - 1) / S;
+            splits.push_back(make_int4(r, nslots, nch, 0));
+            for (int c = 0; c < nch; c++) {
+                const int pb = rp[(size_t)r] + c * S;
+                const int pe = std::min(pb + S, rp[(size_t)r + 1]);
+                tiles.push_back(make_int4(r, -1 - (nslots + c), pb, pe));
+            }
+            nslots += nch;
+            ++r;
+        } else {
+            const int rb = r, p0 = rp[(size_t)r];
+            int re = r + 1;
+            while (re < nr && re - rb < S && rp[(size_t)re + 1] - p0 <= S) ++re;
+            tiles.push_back(make_int4(rb, re, p0, rp[(size_t)re]));
+            r = re;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace dspmm
+
+using namespace dspmm;
+
+struct dspmm_matrix_s {
+    int nr = 0, nc = 0, nnz = 0;
+    int *rowPtr = nullptr, *rowIdx = nullptr, *colIdx = nullptr;
+    float *values = nullptr;
+    std::vector<int> h_rowPtr;
+    int max_len = 0;
+    double mean = 0, cv = 0;
+    size_t base_bytes = 0;
+    std::mutex mu;
+    std::map<int, NbPlan> plans;                                   // by S
+    std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
+    float *carry = nullptr;
+    size_t carry_bytes = 0;
+
+    size_t hbm_bytes() {
+        size_t b = base_bytes + carry_bytes;
+        if (rowIdx) b += sizeof(int) * (size_t)nnz;
+        for (auto &kv : plans) b += kv.second.bytes;
+        return b;
+    }
+};
+
+namespace {
+
+void free_matrix(dspmm_matrix_s *A) {
+    if (!A) return;
+    hipFree(A->rowPtr);
+    hipFree(A->rowIdx);
+    hipFree(A->colIdx);
+    hipFree(A->values);
+    for (auto &kv : A->plans) {
+        hipFree(kv.second.d_tiles);
+        hipFree(kv.second.d_splits);
+    }
+    hipFree(A->carry);
+    delete A;
+}
+
+void compute_stats(dspmm_matrix_s *A) {
+    const auto &rp = A->h_rowPtr;
+    A->max_len = 0;
+    double s = 0, s2 = 0;
+    for (int r = 0; r < A->nr; r++) {
+        const int len = rp[(size_t)r + 1] - rp[(size_t)r];
+        A->max_len = std::max(A->max_len, len);
+        s += len;
+        s2 += (double)len * len;
+    }
+    A->mean = A->nr ? s / A->nr : 0.0;
+    const double var = A->nr ? std::max(0.0, s2 / A->nr - A->mean * A->mean) : 0.0;
+    A->cv = A->mean > 0 ? std::sqrt(var) / A->mean : 0.0;
+}
+
+bool rowptr_ok(const std::vector<int> &rp, int nr, int nnz) {
+    if (rp.size() != (size_t)nr + 1 || rp[0] != 0 || rp[(size_t)nr] != nnz) return false;
+    for (int r = 0; r < nr; r++)
+        if (rp[(size_t)r + 1] < rp[(size_t)r]) return false;
+    return true;
+}
+
+// Row offsets from a sorted COO row array; same result as
+// generateCsrRowPtrArray (spmvspmm_benchmark.cu:41-76) for valid input,
+// and also defined for nnz == 0.
+bool rowptr_from_rows(const int *rows, int nr, int nnz, std::vector<int> &rp) {
+    rp.assign((size_t)nr + 1, 0);
+    for (int i = 0; i < nnz; i++) {
+        if (rows[i] < 0 || rows[i] >= nr) return false;
+        if (i && rows[i] < rows[i - 1]) return false;
+        rp[(size_t)rows[i] + 1]++;
+    }
+    for (int r = 0; r < nr; r++) rp[(size_t)r + 1] += rp[(size_t)r];
+    return true;
+}
+
+template <typename T>
+dspmm_status_t upload(T **dst, const T *src, size_t n, hipMemcpyKind kind, hipStream_t st) {
+    *dst = nullptr;
+    if (n == 0) return DSPMM_STATUS_SUCCESS;
+    dspmm_status_t s = hip_status(hipMalloc((void **)dst, n * sizeof(T)));
+    if (s != DSPMM_STATUS_SUCCESS) return s;
+    if (kind == hipMemcpyHostToDevice) return hip_status(hipMemcpy(*dst, src, n * sizeof(T), kind));
+    return hip_status(hipMemcpyAsync(*dst, src, n * sizeof(T), kind, st));
+}
+
+dspmm_status_t ensure_rowidx(dspmm_matrix_s *A) {
+    if (A->rowIdx || A->nnz == 0) return DSPMM_STATUS_SUCCESS;
+    std::vector<int> rows((size_t)A->nnz);
+    for (int r = 0; r < A->nr; r++)
+        for (int q = A->h_rowPtr[(size_t)r]; q < A->h_rowPtr[(size_t)r + 1]; q++) rows[(size_t)q] = r;
+    return upload(&A->rowIdx, rows.data(), rows.size(), hipMemcpyHostToDevice, nullptr);
+}
+
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out) {
+    auto it = A->plans.find(S);
+    if (it != A->plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
+    std::vector<int4> tiles, splits;
+    int nslots = 0;
+    build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
+    NbPlan p;
+    p.S = S;
+    p.ntiles = (int)tiles.size();
+    p.nsplits = (int)splits.size();
+    p.nslots = nslots;
+    dspmm_status_t st = upload(&p.d_tiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = upload(&p.d_splits, splits.data(), splits.size(), hipMemcpyHostToDevice, nullptr);
+    if (st != DSPMM_STATUS_SUCCESS) {
+        hipFree(p.d_tiles);
+        hipFree(p.d_splits);
+        return st;
+    }
+    p.bytes = sizeof(int4) * (tiles.size() + splits.size());
+    auto res = A->plans.emplace(S, p);
+    *out = &res.first->second;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+int64_t carry_ld(int nv) { return ((int64_t)nv + 7) / 8 * 8; }
+
+// LDS budget of one nnz-tiled workgroup (keeps two workgroups per CU).
+constexpr size_t kMaxTileLds = 64 * 1024;
+
+int default_tile(int lanes) { return std::max(16, std::min(256, 16 * lanes)); }
+
+// Fills unset fields of an NNZ_TILED config for these operands.
+void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,
+                 bool bf16) {
+    const int eb = bf16 ? 2 : 4;
+    const int maxvw = bf16 ? 8 : 4;
+    const int auto_vw = choose_vw(nv, ldb, ldc, B, C, eb, maxvw);
+    if (c.vec <= 0 || c.vec > auto_vw || (auto_vw % c.vec) != 0) c.vec = auto_vw;
+    const int need = pow2_ceil(div_up(nv, c.vec));
+    if (c.lanes <= 0 || c.lanes > 64 || (c.lanes & (c.lanes - 1)) != 0) c.lanes = std::min(64, need);
+    if (c.tile_nnz <= 0) c.tile_nnz = default_tile(c.lanes);
+    if (c.unroll <= 0) c.unroll = 8;
+}
+
+dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, dspmm_dtype_t bt) {
+    (void)A;
+    dspmm_config c{};
+    if (layout == DENSE_ROW_MAJOR || bt == DSPMM_R_16BF) c.variant = DSPMM_VARIANT_NNZ_TILED;
+    else c.variant = DSPMM_VARIANT_CSR_SCALAR;
+    return c;
+}
+
+__global__ void k_zero_2d(float *__restrict__ p, int rows, int cols, int64_t ld) {
+    const int64_t n = (int64_t)rows * cols;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t r = i / cols, c = i % cols;
+        p[r * ld + c] = 0.f;
+    }
+}
+
+dspmm_status_t zero_dense(float *C, int rows, int cols, int64_t ld, hipStream_t st) {
+    if (rows == 0 || cols == 0) return DSPMM_STATUS_SUCCESS;
+    if (ld == cols) return launch_fill_zero(C, sizeof(float) * (size_t)rows * (size_t)cols, st);
+    const int64_t n = (int64_t)rows * cols;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_zero_2d, dim3(blocks), dim3(256), 0, st, C, rows, cols, ld);
+    return launch_status();
+}
+
+// `mid` (may be null) is recorded on the stream after the main kernel of
+// the call (before the split-row fix-up of the nnz-tiled variant).
+dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void *B, int64_t ldb,
+                      dspmm_dtype_t bt, float *C, int64_t ldc, const dspmm_config &cfg,
+                      void *ws, size_t ws_bytes, hipStream_t st, hipEvent_t mid = nullptr) {
+    const bool bf16 = bt == DSPMM_R_16BF;
+    switch (cfg.variant) {
+    case DSPMM_VARIANT_NNZ_TILED: {
+        if (layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
+        dspmm_config c = cfg;
+        complete_nb(c, nv, ldb, ldc, B, C, bf16);
+        if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) return DSPMM_STATUS_NOT_SUPPORTED;
+        NbPlan *plan = nullptr;
+        dspmm_status_t s = get_plan(A, c.tile_nnz, &plan);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        const int64_t ldcar = carry_ld(nv);
+        const size_t need = sizeof(float) * (size_t)plan->nslots * (size_t)ldcar;
+        float *carry = nullptr;
+        if (need > 0) {
+            if (ws) {
+                if (ws_bytes < need) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
+                if (!aligned(ws, 32)) return DSPMM_STATUS_INVALID_VALUE;
+                carry = reinterpret_cast<float *>(ws);
+            } else {
+                if (A->carry_bytes < need) {
+                    hipFree(A->carry);
+                    A->carry = nullptr;
+                    A->carry_bytes = 0;
+                    s = hip_status(hipMalloc((void **)&A->carry, need));
+                    if (s != DSPMM_STATUS_SUCCESS) return s;
+                    A->carry_bytes = need;
+                }
+                carry = A->carry;
+            }
+        }
+        NbLaunch a;
+        a.tiles = plan->d_tiles;
+        a.ntiles = plan->ntiles;
+        a.splits = plan->d_splits;
+        a.nsplits = plan->nsplits;
+        a.rowPtr = A->rowPtr;
+        a.colIdx = A->colIdx;
+        a.values = A->values;
+        a.B = B;
+        a.ldb = ldb;
+        a.bf16 = bf16;
+        a.C = C;
+        a.ldc = ldc;
+        a.carry = carry;
+        a.ldcarry = ldcar;
+        a.nv = nv;
+        a.vw = c.vec;
+        a.lanes = c.lanes;
+        a.slab_w = c.vec * c.lanes;
+        a.nslabs = div_up(nv, a.slab_w);
+        a.S = c.tile_nnz;
+        a.unroll = c.unroll;
+        a.mid_event = mid;
+        return launch_nnz_tiled(a, st);
+    }
+    case DSPMM_VARIANT_CSR_SCALAR:
+    case DSPMM_VARIANT_CSR_VECTOR:
+    case DSPMM_VARIANT_COO_SCALAR:
+    case DSPMM_VARIANT_COO_VECTOR: {
+        if (bf16) return DSPMM_STATUS_NOT_SUPPORTED;
+        const bool rm = layout == DENSE_ROW_MAJOR;
+        int which = 0;
+        switch (cfg.variant) {
+        case DSPMM_VARIANT_CSR_SCALAR: which = rm ? K_SPMM_CSR_SCALAR_ROW : K_SPMM_CSR_SCALAR_COL; break;
+        case DSPMM_VARIANT_CSR_VECTOR: which = rm ? K_SPMM_CSR_VECTOR_ROW : K_SPMM_CSR_VECTOR_COL; break;
+        case DSPMM_VARIANT_COO_SCALAR: which = rm ? K_SPMM_COO_SCALAR_ROW : K_SPMM_COO_SCALAR_COL; break;
+        default: which = rm ? K_SPMM_COO_VECTOR_ROW : K_SPMM_COO_VECTOR_COL; break;
+        }
+        const bool coo = cfg.variant == DSPMM_VARIANT_COO_SCALAR || cfg.variant == DSPMM_VARIANT_COO_VECTOR;
+        if (coo) {
+            dspmm_status_t s = ensure_rowidx(A);
+            if (s != DSPMM_STATUS_SUCCESS) return s;
+            s = rm ? zero_dense(C, A->nr, nv, ldc, st) : zero_dense(C, nv, A->nr, ldc, st);
+            if (s != DSPMM_STATUS_SUCCESS) return s;
+        }
+        dspmm_status_t s = launch_compat(which, A->nr, A->nc, A->nnz, nv, ldb, ldc, A->rowPtr,
+                                         A->rowIdx, A->colIdx, A->values,
+                                         reinterpret_cast<const float *>(B), C, st, cfg.lanes);
+        if (mid) (void)hipEventRecord(mid, st);
+        return s;
+    }
+    }
+    return DSPMM_STATUS_INVALID_VALUE;
+}
+
+dspmm_status_t check_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void *B, int64_t ldb,
+                        dspmm_dtype_t bt, const float *C, int64_t ldc) {
+    if (!A || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
+    if (layout != DENSE_ROW_MAJOR && layout != DENSE_COL_MAJOR) return DSPMM_STATUS_INVALID_VALUE;
+    if (bt != DSPMM_R_32F && bt != DSPMM_R_16BF) return DSPMM_STATUS_INVALID_VALUE;
+    if (bt == DSPMM_R_16BF && layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
+    if (layout == DENSE_ROW_MAJOR) {
+        if (ldb < nv || ldc < nv) return DSPMM_STATUS_INVALID_VALUE;
+    } else {
+        if (ldb < A->nc || ldc < A->nr) return DSPMM_STATUS_INVALID_VALUE;
+    }
+    if (A->nr > 0 && !C) return DSPMM_STATUS_INVALID_VALUE;
+    if (A->nnz > 0 && !B) return DSPMM_STATUS_INVALID_VALUE;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt,
+                     const dspmm_config *cfg) {
+    if (cfg && cfg->variant != DSPMM_VARIANT_AUTO) return *cfg;
+    auto it = A->tuned.find(std::make_tuple((int)layout, nv, (int)bt));
+    if (it != A->tuned.end()) return it->second;
+    return heuristic(A, layout, bt);
+}
+
+float median_of(std::vector<float> v) {
+    if (v.empty()) return 0.f;
+    std::sort(v.begin(), v.end());
+    return v[v.size() / 2];
+}
+
+}  // namespace
+
+extern "C" {
+
+dspmm_status_t dspmm_fill_zero(void *ptr, size_t bytes, dspmm_stream_t stream) {
+    if (bytes && !ptr) return DSPMM_STATUS_INVALID_VALUE;
+    return launch_fill_zero(ptr, bytes, as_hip(stream));
+}
+
+dspmm_status_t dspmm_convert_f32_to_bf16(const float *src, void *dst, int64_t n,
+                                         dspmm_stream_t stream) {
+    if (n < 0 || (n > 0 && (!src || !dst))) return DSPMM_STATUS_INVALID_VALUE;
+    return launch_f32_to_bf16(src, reinterpret_cast<uint16_t *>(dst), n, as_hip(stream));
+}
+
+dspmm_status_t dspmm_matrix_create(dspmm_matrix_t *out, int nr, int nc, int nnz,
+                                   const int *h_rowPtr, const int *h_rowIdx, const int *h_colIdx,
+                                   const float *h_values, unsigned flags) {
+    if (!out) return DSPMM_STATUS_INVALID_VALUE;
+    *out = nullptr;
+    if (nr < 0 || nc < 0 || nnz < 0) return DSPMM_STATUS_INVALID_VALUE;
+    if (nnz > 0 && (!h_colIdx || !h_values)) return DSPMM_STATUS_INVALID_VALUE;
+    if (!h_rowPtr && !h_rowIdx && nnz > 0) return DSPMM_STATUS_INVALID_VALUE;
+    auto *A = new dspmm_matrix_s;
+    A->nr = nr;
+    A->nc = nc;
+    A->nnz = nnz;
+    if (h_rowPtr) {
+        A->h_rowPtr.assign(h_rowPtr, h_rowPtr + (size_t)nr + 1);
+        if (!rowptr_ok(A->h_rowPtr, nr, nnz)) { delete A; return DSPMM_STATUS_INVALID_VALUE; }
+    } else if (!rowptr_from_rows(h_rowIdx, nr, nnz, A->h_rowPtr)) {
+        delete A;
+        return DSPMM_STATUS_INVALID_VALUE;
+    }
+    for (int i = 0; i < nnz; i++)
+        if (h_colIdx[i] < 0 || h_colIdx[i] >= nc) { delete A; return DSPMM_STATUS_INVALID_VALUE; }
+    compute_stats(A);
+    dspmm_status_t st = upload(&A->rowPtr, A->h_rowPtr.data(), (size_t)nr + 1, hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = upload(&A->colIdx, h_colIdx, (size_t)nnz, hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = upload(&A->values, h_values, (size_t)nnz, hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS && (flags & DSPMM_MATRIX_WITH_COO)) {
+        if (h_rowIdx) st = upload(&A->rowIdx, h_rowIdx, (size_t)nnz, hipMemcpyHostToDevice, nullptr);
+        else st = ensure_rowidx(A);
+    }
+    if (st != DSPMM_STATUS_SUCCESS) { free_matrix(A); return st; }
+    A->base_bytes = sizeof(int) * ((size_t)nr + 1 + (size_t)nnz) + sizeof(float) * (size_t)nnz;
+    *out = A;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_matrix_create_device(dspmm_matrix_t *out, int nr, int nc, int nnz,
+                                          const int *d_rowPtr, const int *d_rowIdx,
+                                          const int *d_colIdx, const float *d_values,
+                                          unsigned flags, dspmm_stream_t stream) {
+    if (!out) return DSPMM_STATUS_INVALID_VALUE;
+    *out = nullptr;
+    if (nr < 0 || nc < 0 || nnz < 0 || !d_rowPtr) return DSPMM_STATUS_INVALID_VALUE;
+    if (nnz > 0 && (!d_colIdx || !d_values)) return DSPMM_STATUS_INVALID_VALUE;
+    std::vector<int> rp((size_t)nr + 1);
+    dspmm_status_t st = hip_status(hipStreamSynchronize(as_hip(stream)));
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = hip_status(hipMemcpy(rp.data(), d_rowPtr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (!rowptr_ok(rp, nr, nnz)) return DSPMM_STATUS_INVALID_VALUE;
+    auto *A = new dspmm_matrix_s;
+    A->nr = nr;
+    A->nc = nc;
+    A->nnz = nnz;
+    A->h_rowPtr.swap(rp);
+    compute_stats(A);
+    const hipStream_t s = as_hip(stream);
+    st = upload(&A->rowPtr, d_rowPtr, (size_t)nr + 1, hipMemcpyDeviceToDevice, s);
+    if (st == DSPMM_STATUS_SUCCESS) st = upload(&A->colIdx, d_colIdx, (size_t)nnz, hipMemcpyDeviceToDevice, s);
+    if (st == DSPMM_STATUS_SUCCESS) st = upload(&A->values, d_values, (size_t)nnz, hipMemcpyDeviceToDevice, s);
+    if (st == DSPMM_STATUS_SUCCESS && (flags & DSPMM_MATRIX_WITH_COO)) {
+        if (d_rowIdx) st = upload(&A->rowIdx, d_rowIdx, (size_t)nnz, hipMemcpyDeviceToDevice, s);
+        else st = ensure_rowidx(A);
+    }
+    if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipStreamSynchronize(s));
+    if (st != DSPMM_STATUS_SUCCESS) { free_matrix(A); return st; }
+    A->base_bytes = sizeof(int) * ((size_t)nr + 1 + (size_t)nnz) + sizeof(float) * (size_t)nnz;
+    *out = A;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_matrix_destroy(dspmm_matrix_t A) {
+    if (!A) return DSPMM_STATUS_SUCCESS;
+    hipDeviceSynchronize();   // no launch may still read the arrays
+    free_matrix(A);
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_matrix_describe(dspmm_matrix_t A, dspmm_matrix_desc *d) {
+    if (!A || !d) return DSPMM_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> lk(A->mu);
+    d->nr = A->nr;
+    d->nc = A->nc;
+    d->nnz = A->nnz;
+    d->rowPtr = A->rowPtr;
+    d->rowIdx = A->rowIdx;
+    d->colIdx = A->colIdx;
+    d->values = A->values;
+    d->hbm_bytes = A->hbm_bytes();
+    d->max_row_len = A->max_len;
+    d->mean_row_len = A->mean;
+    d->cv_row_len = A->cv;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_spmm_ex_resolve(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                     dspmm_dtype_t btype, const dspmm_config *cfg,
+                                     dspmm_config *resolved) {
+    if (!A || !resolved || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> lk(A->mu);
+    *resolved = resolve(A, layout, nv, btype, cfg);
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                            dspmm_dtype_t btype, const dspmm_config *cfg,
+                                            size_t *bytes) {
+    if (!A || !bytes || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> lk(A->mu);
+    dspmm_config c = resolve(A, layout, nv, btype, cfg);
+    *bytes = 0;
+    if (c.variant != DSPMM_VARIANT_NNZ_TILED) return DSPMM_STATUS_SUCCESS;
+    // Tile size does not depend on pointers; use aligned placeholders.
+    complete_nb(c, nv, nv, nv, reinterpret_cast<const void *>(256), reinterpret_cast<const void *>(256),
+                btype == DSPMM_R_16BF);
+    NbPlan *plan = nullptr;
+    dspmm_status_t st = get_plan(A, c.tile_nnz, &plan);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(nv);
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                             int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,
+                             const dspmm_config *cfg, void *workspace, size_t workspace_bytes,
+                             dspmm_stream_t stream) {
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (A->nr == 0) return DSPMM_STATUS_SUCCESS;
+    std::lock_guard<std::mutex> lk(A->mu);
+    const dspmm_config c = resolve(A, layout, nv, btype, cfg);
+    return run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, workspace, workspace_bytes,
+                  as_hip(stream));
+}
+
+dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                                  int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,
+                                  const dspmm_config *cfg, int warmup, int reps, float *ms_mean,
+                                  float *ms_per_launch, float *ms_main_per_launch,
+                                  dspmm_stream_t stream) {
+    if (reps < 1 || warmup < 0) return DSPMM_STATUS_INVALID_VALUE;
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (A->nr == 0) return DSPMM_STATUS_INVALID_VALUE;
+    const hipStream_t hs = as_hip(stream);
+    dspmm_config c;
+    {
+        std::lock_guard<std::mutex> lk(A->mu);
+        c = resolve(A, layout, nv, btype, cfg);
+    }
+    for (int i = 0; i < warmup && st == DSPMM_STATUS_SUCCESS; i++) {
+        std::lock_guard<std::mutex> lk(A->mu);
+        st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, nullptr, 0, hs);
+    }
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    st = hip_status(hipStreamSynchronize(hs));
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    std::vector<hipEvent_t> ev((size_t)reps + 1, nullptr), mid((size_t)reps, nullptr);
+    for (auto &e : ev) if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventCreate(&e));
+    for (auto &e : mid) if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventCreate(&e));
+    if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventRecord(ev[0], hs));
+    for (int i = 0; i < reps && st == DSPMM_STATUS_SUCCESS; i++) {
+        {
+            std::lock_guard<std::mutex> lk(A->mu);
+            st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, nullptr, 0, hs, mid[(size_t)i]);
+        }
+        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventRecord(ev[(size_t)i + 1], hs));
+    }
+    if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventSynchronize(ev[(size_t)reps]));
+    if (st == DSPMM_STATUS_SUCCESS) {
+        double sum = 0;
+        for (int i = 0; i < reps && st == DSPMM_STATUS_SUCCESS; i++) {
+            float t = 0, tm = 0;
+            st = hip_status(hipEventElapsedTime(&t, ev[(size_t)i], ev[(size_t)i + 1]));
+            if (st == DSPMM_STATUS_SUCCESS)
+                st = hip_status(hipEventElapsedTime(&tm, ev[(size_t)i], mid[(size_t)i]));
+            sum += t;
+            if (ms_per_launch) ms_per_launch[i] = t;
+            if (ms_main_per_launch) ms_main_per_launch[i] = tm;
+        }
+        if (ms_mean) *ms_mean = (float)(sum / reps);
+    }
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : mid) if (e) (void)hipEventDestroy(e);
+    return st;
+}
+
+dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                              int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc, int reps,
+                              dspmm_config *best, float *best_ms, dspmm_stream_t stream) {
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (reps < 1) reps = 5;
+    // Candidate list: the native nnz-tiled kernel over tile sizes, unrolls
+    // and vector widths, then the four reference algorithm classes.
+    std::vector<dspmm_config> cand;
+    const bool bf16 = btype == DSPMM_R_16BF;
+    if (layout == DENSE_ROW_MAJOR) {
+        dspmm_config base{};
+        base.variant = DSPMM_VARIANT_NNZ_TILED;
+        complete_nb(base, nv, ldb, ldc, B, C, bf16);
+        std::vector<int> vws{base.vec};
+        if (base.vec >= 4) vws.push_back(base.vec / 2);
+        for (int vw : vws) {
+            dspmm_config c{};
+            c.variant = DSPMM_VARIANT_NNZ_TILED;
+            c.vec = vw;
+            complete_nb(c, nv, ldb, ldc, B, C, bf16);
+            const int s0 = default_tile(c.lanes);
+            for (int S : {s0 / 4, s0 / 2, s0, s0 * 2}) {
+                if (S < 4 || S > 1024) continue;
+                if (nnz_tiled_lds_bytes(c.lanes, S) > kMaxTileLds) continue;
+                for (int u : {4, 8}) {
+                    dspmm_config x = c;
+                    x.tile_nnz = S;
+                    x.unroll = u;
+                    cand.push_back(x);
+                }
+            }
+        }
+    }
+    if (!bf16) {
+        for (int v : {DSPMM_VARIANT_CSR_SCALAR, DSPMM_VARIANT_CSR_VECTOR, DSPMM_VARIANT_COO_SCALAR,
+                      DSPMM_VARIANT_COO_VECTOR}) {
+            dspmm_config c{};
+            c.variant = v;
+            cand.push_back(c);
+        }
+    }
+    float best_t = 1e30f;
+    dspmm_config best_c = heuristic(A, layout, btype);
+    std::vector<float> ms((size_t)reps);
+    for (const auto &c : cand) {
+        float mean = 0;
+        st = dspmm_benchmark_ex(A, layout, nv, B, ldb, btype, C, ldc, &c, 1, reps, &mean, ms.data(),
+                                nullptr, stream);
+        if (st == DSPMM_STATUS_NOT_SUPPORTED) continue;
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        const float med = median_of(ms);
+        if (med < best_t) { best_t = med; best_c = c; }
+    }
+    {
+        std::lock_guard<std::mutex> lk(A->mu);
+        A->tuned[std::make_tuple((int)layout, nv, (int)btype)] = best_c;
+    }
+    if (best) *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+"""Python host side of the MI355X SpMV/SpMM library (ctypes over libdspmm.so).
+
+Mirrors the reference's host interface for this path:
+
+* ``SpmvSpmmProblem`` -- same constructor arguments and the same ``run`` /
+  ``benchmark`` / ``dry_run`` methods as the reference's struct
+  (spmvspmm.h:13-50, :202-273).  As in the reference, a non power-of-two
+  ``nv`` prints a message to stderr and returns (``benchmark`` returns -1),
+  and ``dry_run`` also refuses nv > 32 (spmvspmm.h:209-213, :223-227,
+  :252-256).  Where the reference exits the process on a CUDA error
+  (helper.h:25-34) this raises ``DspmmError``.
+* ``DeviceArray`` -- the owning upload of hgpu::CUArray (helper.h:78-154).
+* ``Matrix`` -- the native device-resident CSR handle (any nv, BF16 B,
+  autotuned variant).
+
+Nothing here computes on the CPU: if the HIP library is missing, importing
+works but every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdspmm.so")
+GEN_PATH = os.path.join(_HERE, "lib", "libdspmm_gen.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+# Reference enums (spmvspmm.h:5-10, helper.h:256-264).
+ALG_CSR_SCALAR, ALG_CSR_VECTOR, ALG_COO_SCALAR, ALG_COO_VECTOR = 0, 1, 2, 3
+SPARSE_FORMAT_CSR, SPARSE_FORMAT_COO = 0, 1
+DENSE_ROW_MAJOR, DENSE_COL_MAJOR = 0, 1
+ALG_NAMES = {ALG_CSR_SCALAR: "CSR_SCALAR", ALG_CSR_VECTOR: "CSR_VECTOR",
+             ALG_COO_SCALAR: "COO_SCALAR", ALG_COO_VECTOR: "COO_VECTOR"}
+# CSV column names of the reference sweep (run.sh:8): {SR,PR}_{RB,EB}_{RM,CM}.
+CSV_NAMES = {(ALG_CSR_SCALAR, 0): "SR_RB_RM", (ALG_CSR_VECTOR, 0): "PR_RB_RM",
+             (ALG_COO_SCALAR, 0): "SR_EB_RM", (ALG_COO_VECTOR, 0): "PR_EB_RM",
+             (ALG_CSR_SCALAR, 1): "SR_RB_CM", (ALG_CSR_VECTOR, 1): "PR_RB_CM",
+             (ALG_COO_SCALAR, 1): "SR_EB_CM", (ALG_COO_VECTOR, 1): "PR_EB_CM"}
+
+DSPMM_R_32F, DSPMM_R_16BF = 0, 1
+VARIANT_AUTO, VARIANT_NNZ_TILED = 0, 1
+VARIANT_CSR_SCALAR, VARIANT_CSR_VECTOR, VARIANT_COO_SCALAR, VARIANT_COO_VECTOR = 2, 3, 4, 5
+VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
+                 4: "coo_scalar", 5: "coo_vector"}
+MATRIX_WITH_COO = 0x1
+
+STATUS_SUCCESS = 0
+STATUS_NOT_SUPPORTED = 2
+
+# The 12 reference-named launchers, in spmv.h / spmm.h order.
+LAUNCHERS = ["spmv_csr_scalar", "spmv_coo_scalar", "spmv_csr_vector", "spmv_coo_vector",
+             "spmm_csr_scalar_row", "spmm_csr_vector_row", "spmm_coo_scalar_row",
+             "spmm_coo_vector_row", "spmm_csr_scalar_col", "spmm_csr_vector_col",
+             "spmm_coo_scalar_col", "spmm_coo_vector_col"]
+
+
+class DspmmError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        msg = f"{what}: {status_string(status)} (status {status}"
+        if status == 4 and _lib is not None:
+            msg += f", hipError {_lib.dspmm_last_hip_error()}"
+        super().__init__(msg + ")")
+        self.status = status
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int),
+                ("wave_size", C.c_int), ("clock_khz", C.c_int), ("l2_bytes", C.c_int),
+                ("lds_per_block", C.c_size_t), ("total_mem", C.c_size_t),
+                ("free_mem", C.c_size_t)]
+
+
+class MatrixDesc(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nc", C.c_int), ("nnz", C.c_int), ("rowPtr", C.c_void_p),
+                ("rowIdx", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p),
+                ("hbm_bytes", C.c_size_t), ("max_row_len", C.c_int),
+                ("mean_row_len", C.c_double), ("cv_row_len", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("variant", C.c_int), ("vec", C.c_int), ("lanes", C.c_int),
+                ("tile_nnz", C.c_int), ("unroll", C.c_int)]
+
+    def as_dict(self):
+        return {"variant": VARIANT_NAMES.get(self.variant, self.variant), "vec": self.vec,
+                "lanes": self.lanes, "tile_nnz": self.tile_nnz, "unroll": self.unroll}
+
+
+_lib = None
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_SZ = C.c_size_t
+_ST = C.c_int  # dspmm_status_t
+
+
+def _declare(lib):
+    launcher = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]
+    for name in LAUNCHERS:
+        f = getattr(lib, name)
+        f.argtypes, f.restype = launcher, _ST
+    sig = {
+        "dspmm_get_error_string": ([_ST], C.c_char_p),
+        "dspmm_last_hip_error": ([], _I),
+        "dspmm_version": ([], C.c_char_p),
+        "dspmm_spmm": ([_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P], _ST),
+        "dspmm_device_alloc": ([C.POINTER(C.c_void_p), _SZ], _ST),
+        "dspmm_device_free": ([_P], _ST),
+        "dspmm_memcpy_h2d": ([_P, _P, _SZ], _ST),
+        "dspmm_memcpy_d2h": ([_P, _P, _SZ], _ST),
+        "dspmm_memcpy_d2d": ([_P, _P, _SZ, _P], _ST),
+        "dspmm_fill_zero": ([_P, _SZ, _P], _ST),
+        "dspmm_stream_create": ([C.POINTER(C.c_void_p)], _ST),
+        "dspmm_stream_destroy": ([_P], _ST),
+        "dspmm_stream_synchronize": ([_P], _ST),
+        "dspmm_device_synchronize": ([], _ST),
+        "dspmm_get_device_info": ([C.POINTER(DeviceInfo)], _ST),
+        "dspmm_timer_create": ([C.POINTER(C.c_void_p), _I], _ST),
+        "dspmm_timer_destroy": ([_P], _ST),
+        "dspmm_timer_reset": ([_P], _ST),
+        "dspmm_timer_mark": ([_P, _P], _ST),
+        "dspmm_timer_intervals": ([_P, C.POINTER(C.c_float), _I], _ST),
+        "dspmm_benchmark_compat": ([_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I,
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float), _P], _ST),
+        "dspmm_dry_run_compat": ([_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _ST),
+        "dspmm_matrix_create": ([C.POINTER(C.c_void_p), _I, _I, _I, _P, _P, _P, _P, C.c_uint],
+                                _ST),
+        "dspmm_matrix_create_device": ([C.POINTER(C.c_void_p), _I, _I, _I, _P, _P, _P, _P,
+                                        C.c_uint, _P], _ST),
+        "dspmm_matrix_destroy": ([_P], _ST),
+        "dspmm_matrix_describe": ([_P, C.POINTER(MatrixDesc)], _ST),
+        "dspmm_spmm_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _P, _SZ, _P],
+                          _ST),
+        "dspmm_spmm_ex_workspace_size": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(_SZ)],
+                                         _ST),
+        "dspmm_spmm_ex_resolve": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(Config)], _ST),
+        "dspmm_autotune": ([_P, _I, _I, _P, _I64, _I, _P, _I64, _I, C.POINTER(Config),
+                            C.POINTER(C.c_float), _P], _ST),
+        "dspmm_benchmark_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _I, _I,
+                                C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), _P], _ST),
+        "dspmm_convert_f32_to_bf16": ([_P, _P, _I64, _P], _ST),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes, f.restype = args, res
+
+
+def lib():
+    """The loaded libdspmm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def status_string(status: int) -> str:
+    if _lib is None:
+        return str(status)
+    return _lib.dspmm_get_error_string(status).decode()
+
+
+def check(status: int, what: str):
+    if status != STATUS_SUCCESS:
+        raise DspmmError(status, what)
+
+
+def declared_symbols(header: str | None = None):
+    """Function names declared with DSPMM_API in include/*.h."""
+    import glob
+    import re
+    files = [header] if header else sorted(glob.glob(os.path.join(INCLUDE_DIR, "*.h")))
+    names = []
+    for path in files:
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith("#"))
+        for m in re.finditer(r"DSPMM_API\s+[^;(]*?\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            names.append(m.group(1))
+    return names
+
+
+def ptr(x) -> int:
+    """Device address of a DeviceArray, or an int passed through."""
+    if x is None:
+        return 0
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return int(x)
+
+
+class DeviceArray:
+    """Owning device buffer (hgpu::CUArray): upload from / download to numpy."""
+
+    def __init__(self, nbytes: int, dtype=np.float32, shape=None):
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(nbytes)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        p = C.c_void_p()
+        check(lib().dspmm_device_alloc(C.byref(p), self.nbytes), "device_alloc")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            check(lib().dspmm_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes), "memcpy_h2d")
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float32):
+        shape = tuple(np.atleast_1d(shape))
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        d = cls(n, dtype, shape)
+        d.zero()
+        return d
+
+    def zero(self, stream=None):
+        if self.nbytes:
+            check(lib().dspmm_fill_zero(self.ptr, self.nbytes, stream), "fill_zero")
+            check(lib().dspmm_stream_synchronize(stream), "sync")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().dspmm_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr and _lib is not None:
+            _lib.dspmm_device_free(self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_info() -> dict:
+    info = DeviceInfo()
+    check(lib().dspmm_get_device_info(C.byref(info)), "get_device_info")
+    return {"name": info.name.decode(), "arch": info.arch.decode(),
+            "compute_units": info.compute_units, "wave_size": info.wave_size,
+            "clock_khz": info.clock_khz, "l2_bytes": info.l2_bytes,
+            "total_mem": info.total_mem, "free_mem": info.free_mem}
+
+
+def synchronize():
+    check(lib().dspmm_device_synchronize(), "device_synchronize")
+
+
+class SpmvSpmmProblem:
+    """The reference's problem descriptor (spmvspmm.h:13-50) over the C-ABI.
+
+    Pointers are device addresses (ints) or DeviceArrays, non-owning."""
+
+    def __init__(self, nr, nc, nnz, maxNv, rowPtr, rowIdx, colIdx, values, dnInput, dnOutput,
+                 stream=None):
+        self.nr, self.nc, self.nnz, self.maxNv = int(nr), int(nc), int(nnz), int(maxNv)
+        self.rowPtr, self.rowIdx, self.colIdx = ptr(rowPtr), ptr(rowIdx), ptr(colIdx)
+        self.values, self.dnInput, self.dnOutput = ptr(values), ptr(dnInput), ptr(dnOutput)
+        self.stream = stream
+
+    def _args(self, nv):
+        return (self.nr, self.nc, self.nnz, nv, self.rowPtr, self.rowIdx, self.colIdx,
+                self.values, self.dnInput, self.dnOutput)
+
+    def run(self, layout, nv, alg):
+        """spmvspmm.h:202-218: dispatch one launch (asynchronous)."""
+        if nv & (nv - 1):
+            print(f"nv must be power of 2, less than 32. Got nv = {nv}", file=sys.stderr)
+            return
+        check(lib().dspmm_spmm(alg, layout, *self._args(nv), self.stream), "SpmvSpmmProblem.run")
+
+    def benchmark(self, layout, nv, alg, warmupIters=5, repeatIters=50, per_launch=None):
+        """spmvspmm.h:220-243: mean milliseconds per launch over repeatIters."""
+        if nv & (nv - 1):
+            print(f"nv must be power of 2, less than 32. Got nv = {nv}", file=sys.stderr)
+            return -1
+        mean = C.c_float()
+        arr = None
+        if per_launch is not None:
+            arr = (C.c_float * repeatIters)()
+        check(lib().dspmm_benchmark_compat(alg, layout, *self._args(nv), warmupIters, repeatIters,
+                                           C.byref(mean), arr, self.stream),
+              "SpmvSpmmProblem.benchmark")
+        if per_launch is not None:
+            per_launch.extend(list(arr))
+        return mean.value
+
+    def dry_run(self, layout, nv, alg, warmupIters=0):
+        """spmvspmm.h:245-273: warm-ups, then one profiled launch."""
+        if (nv & (nv - 1)) or nv > 32:
+            print(f"nv must be power of 2, less than 32. Got nv = {nv}", file=sys.stderr)
+            return
+        check(lib().dspmm_dry_run_compat(alg, layout, *self._args(nv), warmupIters, self.stream),
+              "SpmvSpmmProblem.dry_run")
+
+
+def launch(name, nr, nc, nnz, nv, rowPtr, rowIdx, colIdx, values, dnInput, dnOutput,
+           stream=None):
+    """Call one of the 12 reference-named launchers; returns the status."""
+    f = getattr(lib(), name)
+    return f(nr, nc, nnz, nv, ptr(rowPtr), ptr(rowIdx), ptr(colIdx), ptr(values), ptr(dnInput),
+             ptr(dnOutput), stream)
+
+
+def make_config(variant=VARIANT_AUTO, vec=0, lanes=0, tile_nnz=0, unroll=0) -> Config:
+    return Config(variant, vec, lanes, tile_nnz, unroll)
+
+
+class Matrix:
+    """Device-resident CSR matrix (native API)."""
+
+    def __init__(self, nr, nc, rowPtr=None, colIdx=None, values=None, rowIdx=None, coo=False):
+        rowPtr = None if rowPtr is None else np.ascontiguousarray(rowPtr, np.int32)
+        rowIdx = None if rowIdx is None else np.ascontiguousarray(rowIdx, np.int32)
+        colIdx = np.ascontiguousarray(colIdx, np.int32)
+        values = np.ascontiguousarray(values, np.float32)
+        self.nr, self.nc, self.nnz = int(nr), int(nc), int(colIdx.size)
+        h = C.c_void_p()
+        flags = MATRIX_WITH_COO if coo else 0
+        check(lib().dspmm_matrix_create(
+            C.byref(h), self.nr, self.nc, self.nnz,
+            None if rowPtr is None else rowPtr.ctypes.data,
+            None if rowIdx is None else rowIdx.ctypes.data,
+            colIdx.ctypes.data if self.nnz else None,
+            values.ctypes.data if self.nnz else None, flags), "matrix_create")
+        self.handle = h.value
+
+    def describe(self) -> dict:
+        d = MatrixDesc()
+        check(lib().dspmm_matrix_describe(self.handle, C.byref(d)), "matrix_describe")
+        return {"nr": d.nr, "nc": d.nc, "nnz": d.nnz, "hbm_bytes": d.hbm_bytes,
+                "max_row_len": d.max_row_len, "mean_row_len": d.mean_row_len,
+                "cv_row_len": d.cv_row_len, "rowPtr": d.rowPtr, "rowIdx": d.rowIdx,
+                "colIdx": d.colIdx, "values": d.values}
+
+    def spmm(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, ldb=None, ldc=None, btype=DSPMM_R_32F,
+             config: Config | None = None, workspace=None, workspace_bytes=0, stream=None):
+        ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
+        ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
+        st = lib().dspmm_spmm_ex(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out), ldc,
+                                 C.byref(config) if config is not None else None,
+                                 ptr(workspace), workspace_bytes, stream)
+        check(st, "spmm_ex")
+
+    def resolve(self, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, config=None) -> Config:
+        out = Config()
+        check(lib().dspmm_spmm_ex_resolve(self.handle, layout, nv, btype,
+                                          C.byref(config) if config is not None else None,
+                                          C.byref(out)), "spmm_ex_resolve")
+        return out
+
+    def workspace_size(self, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, config=None) -> int:
+        n = C.c_size_t()
+        check(lib().dspmm_spmm_ex_workspace_size(self.handle, layout, nv, btype,
+                                                 C.byref(config) if config is not None else None,
+                                                 C.byref(n)), "spmm_ex_workspace_size")
+        return n.value
+
+    def autotune(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, reps=5,
+                 ldb=None, ldc=None, stream=None):
+        ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
+        ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
+        best, ms = Config(), C.c_float()
+        check(lib().dspmm_autotune(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out), ldc,
+                                   reps, C.byref(best), C.byref(ms), stream), "autotune")
+        return best, ms.value
+
+    def benchmark(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, config=None,
+                  warmup=5, reps=50, ldb=None, ldc=None, stream=None):
+        """Returns (mean ms per call, per-call ms list, per-call main-kernel ms list)."""
+        ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
+        ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
+        mean = C.c_float()
+        arr = (C.c_float * reps)()
+        main = (C.c_float * reps)()
+        check(lib().dspmm_benchmark_ex(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out),
+                                       ldc, C.byref(config) if config is not None else None,
+                                       warmup, reps, C.byref(mean), arr, main, stream),
+              "benchmark_ex")
+        return mean.value, list(arr), list(main)
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.dspmm_matrix_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_bf16_device(src: DeviceArray, stream=None) -> DeviceArray:
+    """Device-side FP32 -> BF16 (RNE) copy of a dense operand."""
+    n = src.nbytes // 4
+    dst = DeviceArray(2 * n, np.uint16, src.shape)
+    check(lib().dspmm_convert_f32_to_bf16(src.ptr, dst.ptr, n, stream), "convert_f32_to_bf16")
+    check(lib().dspmm_stream_synchronize(stream), "sync")
+    return dst

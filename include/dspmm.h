@@ -1,0 +1,303 @@
+/*
+ * dspmm.h -- C-ABI of the MI355X (gfx950) SpMV/SpMM kernel library.
+ *
+ * Drop-in boundary for the reference's kernel family (spmv.h, spmm.h) and
+ * its dispatcher (spmvspmm.h).  Every entry point takes plain pointers and
+ * sizes; streams are passed as `dspmm_stream_t`, which is a hipStream_t
+ * (NULL = the default stream).  No entry point exits the process: errors
+ * come back as dspmm_status_t.
+ *
+ * Data contract (same as the reference, spmvspmm.h:13-50):
+ *   - 0-based int32 indices; COO rows sorted; rowPtr[nr+1] consistent;
+ *   - rowIdx (COO) is read only by the COO (nnz-balanced) algorithms;
+ *   - row-major dense operands are packed with ld = nv, column-major ones
+ *     with ld = nc (input) and ld = nr (output);
+ *   - all pointers are caller-owned device memory; launchers allocate
+ *     nothing and are asynchronous on `stream`.
+ * Output semantics (spmvspmm_benchmark.cu:141-157):
+ *   - CSR (row-balanced) algorithms OVERWRITE every output entry;
+ *   - COO (nnz-balanced) algorithms ACCUMULATE into the output with atomic
+ *     adds, so the caller zeroes it first, as the reference's harness does.
+ *
+ * Every function declared with DSPMM_API is exported by libdspmm.so.
+ */
+#ifndef DSPMM_H
+#define DSPMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSPMM_API __attribute__((visibility("default")))
+
+/* A hipStream_t; NULL selects the default stream. */
+typedef void *dspmm_stream_t;
+
+/* ---------------------------------------------------------------------
+ * Enums with the reference's names and numeric values.
+ * ------------------------------------------------------------------- */
+
+/* spmvspmm.h:5-10 */
+typedef enum SPMV_SPMM_ALG {
+    ALG_CSR_SCALAR = 0, /* row-balanced, sequential reduction   (SR_RB) */
+    ALG_CSR_VECTOR = 1, /* row-balanced, parallel reduction     (PR_RB) */
+    ALG_COO_SCALAR = 2, /* nnz-balanced, sequential reduction   (SR_EB) */
+    ALG_COO_VECTOR = 3  /* nnz-balanced, parallel reduction     (PR_EB) */
+} SPMV_SPMM_ALG;
+
+/* helper.h:256-259 */
+typedef enum SparseFormat { SPARSE_FORMAT_CSR = 0, SPARSE_FORMAT_COO = 1 } SparseFormat;
+
+/* helper.h:261-264 */
+typedef enum DenseLayout { DENSE_ROW_MAJOR = 0, DENSE_COL_MAJOR = 1 } DenseLayout;
+
+typedef enum dspmm_status {
+    DSPMM_STATUS_SUCCESS = 0,
+    DSPMM_STATUS_INVALID_VALUE = 1,     /* bad size, pointer or enum       */
+    DSPMM_STATUS_NOT_SUPPORTED = 2,     /* e.g. nv outside a kernel's range */
+    DSPMM_STATUS_ALLOC_FAILED = 3,      /* device or host allocation failed */
+    DSPMM_STATUS_HIP_ERROR = 4,         /* see dspmm_last_hip_error()       */
+    DSPMM_STATUS_WORKSPACE_TOO_SMALL = 5,
+    DSPMM_STATUS_ADMISSION_REJECTED = 6 /* request exceeds the HBM budget   */
+} dspmm_status_t;
+
+DSPMM_API const char *dspmm_get_error_string(dspmm_status_t status);
+/* hipError_t of the most recent DSPMM_STATUS_HIP_ERROR on this thread. */
+DSPMM_API int dspmm_last_hip_error(void);
+DSPMM_API const char *dspmm_version(void);
+
+/* ---------------------------------------------------------------------
+ * The 12 kernel launchers.  Names are the reference kernels' names
+ * without the `_kernel` suffix and template arguments; the argument list
+ * is the reference's common one (spmv.h:6-18) plus a stream.  The
+ * column-major CSR kernels declare (nv, nnz) swapped in the reference
+ * (spmm.h:274-275, 320-321) but are called with (nnz, nv)
+ * (spmvspmm.h:169,194); these launchers use the call-site order.
+ *
+ * Accepted nv: spmv_*_scalar: 1.  spmv_*_vector: 1, 2, 4 (row-major).
+ * spmm_*_row / spmm_*_col: any nv >= 1 (the reference needs a power of
+ * two; the dispatcher below keeps that rule).
+ * ------------------------------------------------------------------- */
+#define DSPMM_LAUNCHER_ARGS                                                  \
+    int nr, int nc, int nnz, int nv, const int *rowPtr, const int *rowIdx,   \
+        const int *colIdx, const float *values, const float *dnInput,        \
+        float *dnOutput, dspmm_stream_t stream
+
+/* spmv.h:6   spmv_csr_scalar_kernel  -- y = A x, one lane per row.         */
+DSPMM_API dspmm_status_t spmv_csr_scalar(DSPMM_LAUNCHER_ARGS);
+/* spmv.h:44  spmv_coo_scalar_kernel  -- y += A x, nnz chunk per lane.      */
+DSPMM_API dspmm_status_t spmv_coo_scalar(DSPMM_LAUNCHER_ARGS);
+/* spmv.h:95  spmv_csr_vector_kernel  -- lane group per row, reduced.       */
+DSPMM_API dspmm_status_t spmv_csr_vector(DSPMM_LAUNCHER_ARGS);
+/* spmv.h:153 spmv_coo_vector_kernel  -- wave over nnz, segmented reduce.   */
+DSPMM_API dspmm_status_t spmv_coo_vector(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:8   spmm_csr_scalar_row_kernel                                     */
+DSPMM_API dspmm_status_t spmm_csr_scalar_row(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:54  spmm_csr_vector_row_kernel                                     */
+DSPMM_API dspmm_status_t spmm_csr_vector_row(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:125 spmm_coo_scalar_row_kernel                                     */
+DSPMM_API dspmm_status_t spmm_coo_scalar_row(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:180 spmm_coo_vector_row_kernel                                     */
+DSPMM_API dspmm_status_t spmm_coo_vector_row(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:271 spmm_csr_scalar_col_kernel                                     */
+DSPMM_API dspmm_status_t spmm_csr_scalar_col(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:317 spmm_csr_vector_col_kernel                                     */
+DSPMM_API dspmm_status_t spmm_csr_vector_col(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:371 spmm_coo_scalar_col_kernel                                     */
+DSPMM_API dspmm_status_t spmm_coo_scalar_col(DSPMM_LAUNCHER_ARGS);
+/* spmm.h:440 spmm_coo_vector_col_kernel                                     */
+DSPMM_API dspmm_status_t spmm_coo_vector_col(DSPMM_LAUNCHER_ARGS);
+
+/* ---------------------------------------------------------------------
+ * Dispatcher: replaces cudaSpmm<256,32> (spmvspmm.h:53-199) with the same
+ * parameter list plus a stream.  Maps (alg, layout, nv) to a launcher the
+ * way the reference does (spmv_* for nv==1, and for nv in {2,4} with the
+ * row-major vector algorithms).  Keeps the reference callers' rule that
+ * nv is a power of two (spmvspmm.h:209,223) but reports
+ * DSPMM_STATUS_NOT_SUPPORTED instead of printing and returning.
+ * ------------------------------------------------------------------- */
+DSPMM_API dspmm_status_t dspmm_spmm(SPMV_SPMM_ALG alg, DenseLayout layout,
+                                    int nr, int nc, int nnz, int nv,
+                                    const int *rowPtr, const int *rowIdx,
+                                    const int *colIdx, const float *values,
+                                    const float *dnInput, float *dnOutput,
+                                    dspmm_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Device memory and timing: the owning-array upload of hgpu::CUArray
+ * (helper.h:78-154) and the event timer hgpu::GpuTimer (helper.h:158-191)
+ * as plain C calls, so that hosts without a HIP toolchain can drive the
+ * library.
+ * ------------------------------------------------------------------- */
+DSPMM_API dspmm_status_t dspmm_device_alloc(void **ptr, size_t bytes);
+DSPMM_API dspmm_status_t dspmm_device_free(void *ptr);
+/* Synchronous copies (like CUArray::sync_device / sync_host). */
+DSPMM_API dspmm_status_t dspmm_memcpy_h2d(void *dst, const void *src, size_t bytes);
+DSPMM_API dspmm_status_t dspmm_memcpy_d2h(void *dst, const void *src, size_t bytes);
+DSPMM_API dspmm_status_t dspmm_memcpy_d2d(void *dst, const void *src, size_t bytes,
+                                          dspmm_stream_t stream);
+/* Zero-fill with a kernel (safe to record in a captured graph). */
+DSPMM_API dspmm_status_t dspmm_fill_zero(void *ptr, size_t bytes, dspmm_stream_t stream);
+DSPMM_API dspmm_status_t dspmm_stream_create(dspmm_stream_t *stream);
+DSPMM_API dspmm_status_t dspmm_stream_destroy(dspmm_stream_t stream);
+DSPMM_API dspmm_status_t dspmm_stream_synchronize(dspmm_stream_t stream);
+DSPMM_API dspmm_status_t dspmm_device_synchronize(void);
+
+typedef struct dspmm_device_info {
+    char name[64];
+    char arch[32];          /* e.g. "gfx950:sramecc+:xnack-" */
+    int compute_units;
+    int wave_size;
+    int clock_khz;
+    int l2_bytes;
+    size_t lds_per_block;
+    size_t total_mem;
+    size_t free_mem;
+} dspmm_device_info;
+DSPMM_API dspmm_status_t dspmm_get_device_info(dspmm_device_info *info);
+
+/* Event timer: each mark records an event on `stream`; intervals between
+ * consecutive marks are returned in milliseconds after a sync. */
+typedef struct dspmm_timer_s *dspmm_timer_t;
+DSPMM_API dspmm_status_t dspmm_timer_create(dspmm_timer_t *t, int max_marks);
+DSPMM_API dspmm_status_t dspmm_timer_destroy(dspmm_timer_t t);
+DSPMM_API dspmm_status_t dspmm_timer_reset(dspmm_timer_t t);
+DSPMM_API dspmm_status_t dspmm_timer_mark(dspmm_timer_t t, dspmm_stream_t stream);
+/* Waits for the last mark; ms[i] = time between mark i and mark i+1.  */
+DSPMM_API dspmm_status_t dspmm_timer_intervals(dspmm_timer_t t, float *ms, int n);
+
+/* SpmvSpmmProblem::benchmark (spmvspmm.h:220-243) as one C call: `warmup`
+ * untimed dispatcher calls, then `reps` timed ones with an event between
+ * every launch.  ms_per_launch[reps] (may be NULL) gets each launch's time,
+ * *ms_mean the mean over the timed region.  Output is not re-zeroed between
+ * launches (as in the reference). */
+DSPMM_API dspmm_status_t dspmm_benchmark_compat(SPMV_SPMM_ALG alg, DenseLayout layout,
+                                                int nr, int nc, int nnz, int nv,
+                                                const int *rowPtr, const int *rowIdx,
+                                                const int *colIdx, const float *values,
+                                                const float *dnInput, float *dnOutput,
+                                                int warmup, int reps, float *ms_mean,
+                                                float *ms_per_launch, dspmm_stream_t stream);
+
+/* SpmvSpmmProblem::dry_run (spmvspmm.h:245-273): warm-ups, then one launch
+ * bracketed by roctx profiler resume/pause and a named roctx range. */
+DSPMM_API dspmm_status_t dspmm_dry_run_compat(SPMV_SPMM_ALG alg, DenseLayout layout,
+                                              int nr, int nc, int nnz, int nv,
+                                              const int *rowPtr, const int *rowIdx,
+                                              const int *colIdx, const float *values,
+                                              const float *dnInput, float *dnOutput,
+                                              int warmup, dspmm_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Native API (no reference counterpart): an owning device-resident CSR
+ * matrix, any nv >= 1, leading dimensions, FP32 or BF16 dense input,
+ * overwrite semantics, a load-balanced plan and a per-matrix autotuner.
+ * ------------------------------------------------------------------- */
+typedef struct dspmm_matrix_s *dspmm_matrix_t;
+
+typedef enum dspmm_dtype { DSPMM_R_32F = 0, DSPMM_R_16BF = 1 } dspmm_dtype_t;
+
+/* Upload flags. */
+#define DSPMM_MATRIX_WITH_COO 0x1u  /* also keep a device rowIdx array */
+
+/* Upload: replaces the four CUArray uploads of the sparse operand
+ * (spmvspmm_benchmark.cu:369-376).  h_rowPtr may be NULL when h_rowIdx
+ * (sorted) is given: rowPtr is then built as generateCsrRowPtrArray does
+ * (spmvspmm_benchmark.cu:41-76), including for nnz == 0. */
+DSPMM_API dspmm_status_t dspmm_matrix_create(dspmm_matrix_t *A, int nr, int nc, int nnz,
+                                             const int *h_rowPtr, const int *h_rowIdx,
+                                             const int *h_colIdx, const float *h_values,
+                                             unsigned flags);
+/* Same, from arrays already in device memory (copied, so the caller keeps
+ * ownership of its buffers).  d_rowIdx may be NULL. */
+DSPMM_API dspmm_status_t dspmm_matrix_create_device(dspmm_matrix_t *A, int nr, int nc,
+                                                    int nnz, const int *d_rowPtr,
+                                                    const int *d_rowIdx,
+                                                    const int *d_colIdx,
+                                                    const float *d_values, unsigned flags,
+                                                    dspmm_stream_t stream);
+DSPMM_API dspmm_status_t dspmm_matrix_destroy(dspmm_matrix_t A);
+
+typedef struct dspmm_matrix_desc {
+    int nr, nc, nnz;
+    const int *rowPtr;     /* device */
+    const int *rowIdx;     /* device, NULL unless uploaded with COO */
+    const int *colIdx;     /* device */
+    const float *values;   /* device */
+    size_t hbm_bytes;      /* device bytes owned by the handle (incl. plans) */
+    int max_row_len;
+    double mean_row_len;
+    double cv_row_len;     /* coefficient of variation of row lengths */
+} dspmm_matrix_desc;
+DSPMM_API dspmm_status_t dspmm_matrix_describe(dspmm_matrix_t A, dspmm_matrix_desc *d);
+
+/* Kernel variants of the native path. */
+typedef enum dspmm_variant {
+    DSPMM_VARIANT_AUTO = 0,     /* cached autotune result, else heuristic */
+    DSPMM_VARIANT_NNZ_TILED = 1,/* nnz-balanced row-aligned tiles, sequential
+                                   per-row sums, deterministic, overwrite  */
+    DSPMM_VARIANT_CSR_SCALAR = 2,
+    DSPMM_VARIANT_CSR_VECTOR = 3,
+    DSPMM_VARIANT_COO_SCALAR = 4,
+    DSPMM_VARIANT_COO_VECTOR = 5
+} dspmm_variant_t;
+
+typedef struct dspmm_config {
+    int variant;    /* dspmm_variant_t */
+    int vec;        /* floats per lane along N: 1, 2, 4 (8 for BF16); 0 = auto */
+    int lanes;      /* lanes per row slab (power of two <= 64); 0 = auto */
+    int tile_nnz;   /* NNZ_TILED: max nonzeros per tile; 0 = auto */
+    int unroll;     /* NNZ_TILED: gathers in flight per lane group; 0 = auto */
+} dspmm_config;
+
+/* C = A * B, overwriting C (nv columns; ldb/ldc in elements, >= nv for
+ * row-major, >= nc / nr for column-major).  btype selects FP32 or BF16 B
+ * storage (BF16: row-major only); accumulation is always FP32.  cfg may be
+ * NULL (auto).  `workspace` may be NULL: the handle's own split-row buffer
+ * is then used, so calls on one handle must then not overlap in time. */
+DSPMM_API dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                       const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                       float *C, int64_t ldc, const dspmm_config *cfg,
+                                       void *workspace, size_t workspace_bytes,
+                                       dspmm_stream_t stream);
+DSPMM_API dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout,
+                                                      int nv, dspmm_dtype_t btype,
+                                                      const dspmm_config *cfg,
+                                                      size_t *bytes);
+/* Resolve what DSPMM_VARIANT_AUTO would run for these arguments. */
+DSPMM_API dspmm_status_t dspmm_spmm_ex_resolve(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                               dspmm_dtype_t btype, const dspmm_config *cfg,
+                                               dspmm_config *resolved);
+
+/* Autotuner: times the candidate variants and tile shapes on the given
+ * operands (C is overwritten), caches the fastest in the handle for
+ * (layout, nv, btype) and returns it. */
+DSPMM_API dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                        const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                        float *C, int64_t ldc, int reps,
+                                        dspmm_config *best, float *best_ms,
+                                        dspmm_stream_t stream);
+
+/* Timed loop over dspmm_spmm_ex (warmup untimed, then reps launches with an
+ * event between launches).  ms_per_launch[reps] gets each call's time;
+ * ms_main_per_launch[reps] (may be NULL) the time of its main kernel only,
+ * i.e. without the split-row combine that follows it (an event is recorded
+ * between the two; no work is skipped). */
+DSPMM_API dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                            const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                            float *C, int64_t ldc, const dspmm_config *cfg,
+                                            int warmup, int reps, float *ms_mean,
+                                            float *ms_per_launch, float *ms_main_per_launch,
+                                            dspmm_stream_t stream);
+
+/* FP32 -> BF16 (round to nearest even) on the device; dst holds n uint16. */
+DSPMM_API dspmm_status_t dspmm_convert_f32_to_bf16(const float *src, void *dst, int64_t n,
+                                                   dspmm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSPMM_H */

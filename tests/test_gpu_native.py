@@ -1,0 +1,191 @@
+"""GPU parity of the native path (dspmm_spmm_ex / Matrix): the nnz-tiled
+kernel over tile shapes, the reference algorithm classes reached through
+the native API, any nv, leading dimensions, BF16-stored B, the autotuner
+and the caller-provided workspace."""
+import numpy as np
+import pytest
+
+import dspmm as D
+import oracle as O
+import workloads as W
+from gpu_common import assert_bf16, assert_bitwise, assert_fp32, bits, case, note
+
+pytestmark = pytest.mark.gpu
+
+RM, CM = D.DENSE_ROW_MAJOR, D.DENSE_COL_MAJOR
+_MATS = {}
+
+
+def matrix(name):
+    if name not in _MATS:
+        c = case(name)
+        _MATS[name] = (c, D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values))
+    return _MATS[name]
+
+
+def split_rows(c, S):
+    return c.rowlen > S
+
+
+def run_native(c, m, nv, layout=RM, config=None, btype=D.DSPMM_R_32F, B=None):
+    c.fill_output(np.nan)
+    m.spmm(B if B is not None else c.d_B, c.d_C, nv, layout=layout, config=config, btype=btype)
+    D.synchronize()
+    return c.output(nv)
+
+
+@pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20"])
+@pytest.mark.parametrize("S,unroll", [(16, 4), (64, 8), (256, 8), (1024, 4)])
+def test_nnz_tiled_tile_shapes(name, S, unroll):
+    c, m = matrix(name)
+    for nv in (1, 2, 3, 8, 32, 64):
+        if nv > c.maxnv:
+            continue
+        cfg = D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=S, unroll=unroll)
+        what = f"{name} nnz_tiled S={S} U={unroll} nv={nv}"
+        note(what)
+        try:
+            out = run_native(c, m, nv, config=cfg)
+        except D.DspmmError as e:
+            assert e.status == D.STATUS_NOT_SUPPORTED   # LDS budget for tiny lanes x big S
+            continue
+        h32, c64, ab = c.ref(RM, nv)
+        split = np.repeat(split_rows(c, S), nv)
+        # rows that fit a tile are summed exactly like the host loop
+        assert_bitwise(out[~split], h32[~split], what + " (unsplit rows)")
+        assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), what, h32)
+
+
+@pytest.mark.parametrize("vec", [1, 2, 4])
+def test_nnz_tiled_vector_widths_and_lanes(vec):
+    c, m = matrix("zipf_mid")
+    for nv in (4, 12, 128):
+        for lanes in (0, 64):
+            cfg = D.make_config(D.VARIANT_NNZ_TILED, vec=vec, lanes=lanes, tile_nnz=128)
+            what = f"zipf_mid vec={vec} lanes={lanes} nv={nv}"
+            note(what)
+            out = run_native(c, m, nv, config=cfg)
+            h32, c64, ab = c.ref(RM, nv)
+            split = np.repeat(split_rows(c, 128), nv)
+            assert_bitwise(out[~split], h32[~split], what)
+            assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), what, h32)
+
+
+@pytest.mark.parametrize("variant", [D.VARIANT_CSR_SCALAR, D.VARIANT_CSR_VECTOR,
+                                     D.VARIANT_COO_SCALAR, D.VARIANT_COO_VECTOR])
+@pytest.mark.parametrize("layout", [RM, CM], ids=["RM", "CM"])
+def test_reference_classes_through_native_api(variant, layout):
+    """Any nv (not only powers of two) and overwrite semantics for all."""
+    c, m = matrix("zipf_mid")
+    for nv in (1, 3, 5, 33, 100):
+        what = f"zipf_mid {D.VARIANT_NAMES[variant]} layout={layout} nv={nv}"
+        note(what)
+        out = run_native(c, m, nv, layout=layout, config=D.make_config(variant))
+        h32, c64, ab = c.ref(layout, nv)
+        if variant == D.VARIANT_CSR_SCALAR:
+            assert_bitwise(out, h32, what)
+        else:
+            assert_fp32(out, c64, ab, c.rowlen_flat(layout, nv), what, h32)
+
+
+def test_leading_dimensions():
+    c, m = matrix("zipf_mid")
+    nv, ldb, ldc = 24, 40, 36
+    B = W.gen_dense(77, c.nc * ldb).reshape(c.nc, ldb)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.from_numpy(np.full((c.nr, ldc), -3.0, np.float32))
+    for variant in (D.VARIANT_NNZ_TILED, D.VARIANT_CSR_SCALAR, D.VARIANT_COO_VECTOR):
+        note(f"ld variant={variant}")
+        d_C = D.DeviceArray.from_numpy(np.full((c.nr, ldc), -3.0, np.float32))
+        m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc, config=D.make_config(variant))
+        D.synchronize()
+        out = d_C.numpy()
+        assert np.all(out[:, nv:] == -3.0), "columns beyond nv must be left alone"
+        dense = np.ascontiguousarray(B[:, :nv])
+        c64, ab = O.spmm_fp64(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, dense)
+        assert_fp32(np.ascontiguousarray(out[:, :nv]).reshape(-1), c64.reshape(-1),
+                    ab.reshape(-1), np.repeat(c.rowlen, nv), "ld")
+
+
+@pytest.mark.parametrize("name", ["zipf_mid", "stencil_20", "edge"])
+def test_bf16_dense_input(name):
+    c, m = matrix(name)
+    nv = 64 if c.maxnv >= 64 else c.maxnv
+    d_B16 = D.to_bf16_device(c.d_B)
+    dense = c.dense(nv)
+    rounded = O.round_bf16(dense)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, rounded).reshape(-1)
+    c64, ab = O.spmm_fp64(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, dense)
+    for S in (64, 256):
+        what = f"{name} bf16 S={S}"
+        note(what)
+        out = run_native(c, m, nv, config=D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=S),
+                         btype=D.DSPMM_R_16BF, B=d_B16)
+        split = np.repeat(split_rows(c, S), nv)
+        # = the host loop on the BF16-rounded B, bit for bit, on unsplit rows
+        assert_bitwise(out[~split], h32r[~split], what)
+        assert_bf16(out, c64.reshape(-1), ab.reshape(-1), what)
+
+
+def test_device_conversion_matches_rne():
+    c, _ = matrix("c1")
+    d16 = D.to_bf16_device(c.d_B)
+    got = d16.numpy().astype(np.uint32) << 16
+    want = O.round_bf16(c.B).view(np.uint32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_autotune_then_auto_is_correct():
+    c, m = matrix("zipf_mid")
+    for nv, layout in ((32, RM), (1, RM), (8, CM)):
+        note(f"autotune nv={nv} layout={layout}")
+        best, ms = m.autotune(c.d_B, c.d_C, nv, layout=layout, reps=3)
+        assert ms > 0 and best.variant != D.VARIANT_AUTO
+        resolved = m.resolve(nv, layout=layout)
+        assert resolved.as_dict() == best.as_dict()
+        out = run_native(c, m, nv, layout=layout)
+        h32, c64, ab = c.ref(layout, nv)
+        assert_fp32(out, c64, ab, c.rowlen_flat(layout, nv), f"auto {best.as_dict()}", h32)
+
+
+def test_caller_workspace_and_deterministic_repeat():
+    c, m = matrix("edge")               # has split rows
+    nv = 16
+    cfg = D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=64)
+    need = m.workspace_size(nv, config=cfg)
+    assert need > 0
+    ws = D.DeviceArray(need, np.uint8)
+    outs = []
+    for _ in range(3):
+        c.fill_output(np.nan)
+        m.spmm(c.d_B, c.d_C, nv, config=cfg, workspace=ws, workspace_bytes=need)
+        D.synchronize()
+        outs.append(c.output(nv).copy())
+    assert all(np.array_equal(bits(outs[0]), bits(o)) for o in outs[1:])
+    with pytest.raises(D.DspmmError) as e:
+        m.spmm(c.d_B, c.d_C, nv, config=cfg, workspace=ws, workspace_bytes=need - 4)
+    assert e.value.status == 5
+
+
+def test_c2_native_auto_all_n():
+    c, m = matrix("c2")
+    for nv in (1, 32, 128):
+        note(f"c2 native nv={nv}")
+        out = run_native(c, m, nv)
+        h32, c64, ab = c.ref(RM, nv)
+        assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), f"c2 native nv={nv}", h32)
+
+
+def test_matrix_from_device_arrays_and_describe():
+    c = case("zipf_mid")
+    import ctypes as C
+    h = C.c_void_p()
+    D.check(D.lib().dspmm_matrix_create_device(C.byref(h), c.nr, c.nc, c.nnz, c.d_rp.ptr, 0,
+                                               c.d_ci.ptr, c.d_v.ptr, 0, None), "create_device")
+    d = D.MatrixDesc()
+    D.check(D.lib().dspmm_matrix_describe(h.value, C.byref(d)), "describe")
+    assert (d.nr, d.nc, d.nnz) == (c.nr, c.nc, c.nnz)
+    assert d.max_row_len == int(c.rowlen.max())
+    assert abs(d.mean_row_len - c.rowlen.mean()) < 1e-9
+    assert d.hbm_bytes >= 4 * (c.nr + 1 + 2 * c.nnz)
+    D.lib().dspmm_matrix_destroy(h.value)
