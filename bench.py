@@ -43,16 +43,18 @@ def cpu_model():
 
 def traffic_from_profile(kernel_prefix, workload_tag):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the
-    copy-kernel calibration factor recorded in the same file); None if absent."""
+    PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    copy-kernel calibration in the same file); None if absent."""
     path = os.path.join(ROOT, "profiles", f"pmc_{workload_tag}.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        k = d["kernels"][kernel_prefix]
-        return int(round((k["FETCH_SIZE_KiB"] + k["WRITE_SIZE_KiB"]) * 1024 * d["calibration"]))
+        name = d.get("main_kernel")
+        if name and name.startswith(kernel_prefix):
+            return int(round(d["kernels"][name]["traffic_bytes"]))
     except (OSError, KeyError, ValueError, TypeError):
-        return None
+        pass
+    return None
 
 
 def run_native(D, m, d_B, d_C, nv, warmup, steps, tune_reps=5):
@@ -65,6 +67,46 @@ def run_native(D, m, d_B, d_C, nv, warmup, steps, tune_reps=5):
     return best, mean, per, main, wall
 
 
+def bench_batched(D, W, np, count, reps=20):
+    """Config 4: `count` independent matrices (M=K ~ U{256..16384}, LogNormal
+    mean row length, N ~ U{16..256}), each autotuned, then run (i) as one
+    persistent launch, (ii) as a replayed hipGraph of it, (iii) as one plain
+    launch per matrix.  Requests per second = count / time of one pass."""
+    t = time.perf_counter()
+    m, d, n = W.c4_params(count)
+    items, keep = [], []
+    nnz_total = bytes_total = flops_total = 0
+    for i in range(count):
+        a = W.c4_matrix(i, m[i], d[i])
+        nv = int(n[i])
+        mat = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+        d_B = D.DeviceArray.from_numpy(W.dense_b(4000 + i, a.nc, nv))
+        d_C = D.DeviceArray.zeros((a.nr, nv))
+        mat.autotune(d_B, d_C, nv, reps=2)
+        items.append((mat, nv, d_B, d_C))
+        keep.append(a.nnz)
+        nnz_total += a.nnz
+        bytes_total += W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv)
+        flops_total += W.flops(a.nnz, nv)
+    log(f"batched: {count} matrices, nnz {nnz_total}, built+tuned in {time.perf_counter() - t:.1f}s")
+    b = D.Batch(items)
+    info = b.describe()
+    res = {"matrices": count, "nnz_total": nnz_total, "algorithmic_bytes": bytes_total,
+           "schedule": {k: info[k] for k in ("units", "segments", "splits", "blocks")}}
+    for name, mode in (("persistent", D.BATCH_PERSISTENT), ("graph", D.BATCH_GRAPH),
+                       ("plain", D.BATCH_PLAIN)):
+        ms, per = b.benchmark(mode, warmup=3, reps=reps)
+        per = np.asarray(per)
+        res[name] = {"ms_mean": ms, "ms_p50": float(np.percentile(per, 50)),
+                     "ms_p99": float(np.percentile(per, 99)), "req_per_s": count / (ms * 1e-3),
+                     "GFLOP/s": flops_total / (ms * 1e-3) / 1e9,
+                     "GB/s_algorithmic": bytes_total / (ms * 1e-3) / 1e9}
+    res["speedup_graph_vs_plain"] = res["plain"]["ms_mean"] / res["graph"]["ms_mean"]
+    res["speedup_persistent_vs_plain"] = res["plain"]["ms_mean"] / res["persistent"]["ms_mean"]
+    b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +116,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the cpu_baseline sample")
     ap.add_argument("--no-extra", action="store_true", help="skip N=1/N=32/reference sweep")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (c4) leg")
+    ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: the SpMM path here runs on a single GPU and "
@@ -138,6 +182,10 @@ def main():
                 sweep[D.CSV_NAMES[(alg, layout)]] = fl / (t_ms * 1e-3) / 1e9
         extra[f"reference_algorithms_N{nv}_GFLOP/s"] = sweep
         log(f"reference algorithms N={nv}: {sweep}")
+
+    if not args.no_batch:
+        extra["batched_c4"] = bench_batched(D, W, np, args.batch_count)
+        log(f"batched: {extra['batched_c4']}")
 
     # CPU baseline: the restated reference host loop, one thread, bounded sample.
     import importlib

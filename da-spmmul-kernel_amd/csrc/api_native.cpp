@@ -12,18 +12,10 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "matrix.hpp"
 #include "native.hpp"
 
 namespace dspmm {
-namespace {
-
-struct NbPlan {
-    int S = 0;
-    int ntiles = 0, nsplits = 0, nslots = 0;
-    int4 *d_tiles = nullptr;
-    int4 *d_splits = nullptr;
-    size_t bytes = 0;
-};
 
 // Cuts rows [0, nr) into tiles of <= S nonzeros and <= S rows that start and
 // end on row boundaries; a row longer than S becomes ceil(len/S) chunks of
@@ -38,7 +30,7 @@ void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &t
         const int len = rp[(size_t)r + 1] - rp[(size_t)r];
         if (len > S) {
             const int nch = (len + S - 1) / S;
-            splits.push_back(make_int4(r, nslots, nch, 0));
+            splits.push_back(make_int4(0, r, nslots, nch));
             for (int c = 0; c < nch; c++) {
                 const int pb = rp[(size_t)r] + c * S;
                 const int pe = std::min(pb + S, rp[(size_t)r + 1]);
@@ -56,34 +48,11 @@ void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &t
     }
 }
 
-}  // namespace
 }  // namespace dspmm
 
 using namespace dspmm;
 
-struct dspmm_matrix_s {
-    int nr = 0, nc = 0, nnz = 0;
-    int *rowPtr = nullptr, *rowIdx = nullptr, *colIdx = nullptr;
-    float *values = nullptr;
-    std::vector<int> h_rowPtr;
-    int max_len = 0;
-    double mean = 0, cv = 0;
-    size_t base_bytes = 0;
-    std::mutex mu;
-    std::map<int, NbPlan> plans;                                   // by S
-    std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
-    float *carry = nullptr;
-    size_t carry_bytes = 0;
-
-    size_t hbm_bytes() {
-        size_t b = base_bytes + carry_bytes;
-        if (rowIdx) b += sizeof(int) * (size_t)nnz;
-        for (auto &kv : plans) b += kv.second.bytes;
-        return b;
-    }
-};
-
-namespace {
+namespace dspmm {
 
 void free_matrix(dspmm_matrix_s *A) {
     if (!A) return;
@@ -173,15 +142,15 @@ dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out) {
         return st;
     }
     p.bytes = sizeof(int4) * (tiles.size() + splits.size());
-    auto res = A->plans.emplace(S, p);
+    p.h_tiles.swap(tiles);
+    p.h_splits.swap(splits);
+    auto res = A->plans.emplace(S, std::move(p));
     *out = &res.first->second;
     return DSPMM_STATUS_SUCCESS;
 }
 
 int64_t carry_ld(int nv) { return ((int64_t)nv + 7) / 8 * 8; }
 
-// LDS budget of one nnz-tiled workgroup (keeps two workgroups per CU).
-constexpr size_t kMaxTileLds = 64 * 1024;
 
 int default_tile(int lanes) { return std::max(16, std::min(256, 16 * lanes)); }
 
@@ -260,26 +229,30 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
             }
         }
         NbLaunch a;
-        a.tiles = plan->d_tiles;
+        ItemDesc &d = a.item;
+        d.tiles = plan->d_tiles;
+        d.rowPtr = A->rowPtr;
+        d.colIdx = A->colIdx;
+        d.values = A->values;
+        d.B = B;
+        d.C = C;
+        d.carry = carry;
+        d.ldb = ldb;
+        d.ldc = ldc;
+        d.ldcarry = ldcar;
+        d.nv = nv;
+        d.slab_w = c.vec * c.lanes;
+        d.S = c.tile_nnz;
+        d.cls = item_class(c.vec, c.lanes);
+        d.ntiles = plan->ntiles;
+        d.nslabs = div_up(nv, d.slab_w);
         a.ntiles = plan->ntiles;
         a.splits = plan->d_splits;
         a.nsplits = plan->nsplits;
-        a.rowPtr = A->rowPtr;
-        a.colIdx = A->colIdx;
-        a.values = A->values;
-        a.B = B;
-        a.ldb = ldb;
         a.bf16 = bf16;
-        a.C = C;
-        a.ldc = ldc;
-        a.carry = carry;
-        a.ldcarry = ldcar;
-        a.nv = nv;
         a.vw = c.vec;
         a.lanes = c.lanes;
-        a.slab_w = c.vec * c.lanes;
-        a.nslabs = div_up(nv, a.slab_w);
-        a.S = c.tile_nnz;
+        a.nslabs = d.nslabs;
         a.unroll = c.unroll;
         a.mid_event = mid;
         return launch_nnz_tiled(a, st);
@@ -344,7 +317,7 @@ float median_of(std::vector<float> v) {
     return v[v.size() / 2];
 }
 
-}  // namespace
+}  // namespace dspmm
 
 extern "C" {
 

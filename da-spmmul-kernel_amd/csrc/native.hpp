@@ -1,4 +1,4 @@
-// native.hpp -- host-side interface of the native kernels.
+// native.hpp -- host/device interface of the native kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,34 +8,57 @@
 
 namespace dspmm {
 
-// Arguments of one nnz-tiled SpMM launch (plus its split-row fix-up).
+// One SpMM of the nnz-tiled path: a matrix's tile plan and its operands.
+// Passed by value to the single-matrix kernels and as a device table to
+// the batched kernels.
+struct ItemDesc {
+    const int4 *tiles;     // {row_begin, row_end | -1-slot, p_begin, p_end}
+    const int *rowPtr;
+    const int *colIdx;
+    const float *values;
+    const void *B;         // float or BF16 (uint16)
+    float *C;
+    float *carry;          // split-row partial sums: slots x ldcarry floats
+    int64_t ldb, ldc, ldcarry;
+    int nv;
+    int slab_w;            // columns per group pass = lanes * vw
+    int S;                 // tile size of the plan (max nonzeros / rows per tile)
+    int cls;               // 8 * log2(vw) + log2(lanes)
+    int ntiles;
+    int nslabs;
+};
+
+// Arguments of one single-matrix nnz-tiled launch (plus its split combine).
 struct NbLaunch {
-    const int4 *tiles = nullptr;   // {row_begin, row_end | -1-slot, p_begin, p_end}
+    ItemDesc item{};
     int ntiles = 0;
-    const int4 *splits = nullptr;  // {row, first_slot, nchunks, 0}
+    const int4 *splits = nullptr;  // {0, row, first_slot, nchunks}
     int nsplits = 0;
-    const int *rowPtr = nullptr;
-    const int *colIdx = nullptr;
-    const float *values = nullptr;
-    const void *B = nullptr;
-    int64_t ldb = 0;
     bool bf16 = false;
-    float *C = nullptr;
-    int64_t ldc = 0;
-    float *carry = nullptr;        // nslots x ldcarry floats
-    int64_t ldcarry = 0;
-    int nv = 0;
     int vw = 1;
     int lanes = 1;
-    int slab_w = 1;
     int nslabs = 1;
-    int S = 64;
     int unroll = 8;
-    hipEvent_t mid_event = nullptr;  // recorded between the main kernel and the fix-up
+    hipEvent_t mid_event = nullptr;  // recorded between the main kernel and the combine
+};
+
+// Arguments of one batched launch.
+struct BatchLaunch {
+    const ItemDesc *items = nullptr;   // device table
+    const int4 *segs = nullptr;        // per-XCD work segments
+    const int *xoff = nullptr;         // 9 segment offsets + 8 unit counts
+    int lds_ints_per_wave = 0;
+    int nblocks = 0;
+    const int4 *splits = nullptr;      // {item, row, first_slot, nchunks}
+    int nsplits = 0;
+    bool bf16 = false;
+    hipEvent_t mid_event = nullptr;
 };
 
 dspmm_status_t launch_nnz_tiled(const NbLaunch &a, hipStream_t st);
+dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st);
 size_t nnz_tiled_lds_bytes(int lanes, int S);
+int item_class(int vw, int lanes);
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
 dspmm_status_t launch_f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t st);
 

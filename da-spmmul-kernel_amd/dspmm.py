@@ -90,6 +90,17 @@ class Config(C.Structure):
                 "lanes": self.lanes, "tile_nnz": self.tile_nnz, "unroll": self.unroll}
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("A", C.c_void_p), ("nv", C.c_int), ("B", C.c_void_p), ("ldb", C.c_int64),
+                ("C", C.c_void_p), ("ldc", C.c_int64)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int), ("units", C.c_int), ("segments", C.c_int),
+                ("splits", C.c_int), ("blocks", C.c_int), ("units_per_xcd", C.c_int * 8),
+                ("table_bytes", C.c_size_t), ("carry_bytes", C.c_size_t)]
+
+
 _lib = None
 _P = C.c_void_p
 _I = C.c_int
@@ -144,6 +155,15 @@ def _declare(lib):
                                 C.POINTER(C.c_float), C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _P], _ST),
         "dspmm_convert_f32_to_bf16": ([_P, _P, _I64, _P], _ST),
+        "dspmm_batch_create": ([C.POINTER(C.c_void_p), C.POINTER(BatchItem), _I, _I, _I], _ST),
+        "dspmm_batch_destroy": ([_P], _ST),
+        "dspmm_batch_describe": ([_P, C.POINTER(BatchInfo)], _ST),
+        "dspmm_batch_run": ([_P, _P], _ST),
+        "dspmm_batch_graph_launch": ([_P, _P], _ST),
+        "dspmm_batch_run_plain": ([_P, _P], _ST),
+        "dspmm_batch_set_operands": ([_P, _I, _P, _P, _P], _ST),
+        "dspmm_benchmark_batch": ([_P, _I, _I, _I, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   _P], _ST),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -412,3 +432,66 @@ def to_bf16_device(src: DeviceArray, stream=None) -> DeviceArray:
     check(lib().dspmm_convert_f32_to_bf16(src.ptr, dst.ptr, n, stream), "convert_f32_to_bf16")
     check(lib().dspmm_stream_synchronize(stream), "sync")
     return dst
+
+
+BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PLAIN = 0, 1, 2
+
+
+class Batch:
+    """Many row-major SpMMs in one persistent launch (dspmm_batch_*).
+
+    items: sequence of (Matrix, nv, B, C) or (Matrix, nv, B, C, ldb, ldc)."""
+
+    def __init__(self, items, btype=DSPMM_R_32F, blocks_per_cu=0):
+        arr = (BatchItem * max(1, len(items)))()
+        self._keep = []
+        for i, it in enumerate(items):
+            m, nv, B, Cm = it[:4]
+            ldb = it[4] if len(it) > 4 else nv
+            ldc = it[5] if len(it) > 5 else nv
+            arr[i] = BatchItem(m.handle, nv, ptr(B), ldb, ptr(Cm), ldc)
+            self._keep.append(m)
+        h = C.c_void_p()
+        check(lib().dspmm_batch_create(C.byref(h), arr, len(items), btype, blocks_per_cu),
+              "batch_create")
+        self.handle = h.value
+        self.count = len(items)
+
+    def describe(self) -> dict:
+        info = BatchInfo()
+        check(lib().dspmm_batch_describe(self.handle, C.byref(info)), "batch_describe")
+        return {"count": info.count, "units": info.units, "segments": info.segments,
+                "splits": info.splits, "blocks": info.blocks,
+                "units_per_xcd": list(info.units_per_xcd), "table_bytes": info.table_bytes,
+                "carry_bytes": info.carry_bytes}
+
+    def run(self, stream=None):
+        check(lib().dspmm_batch_run(self.handle, stream), "batch_run")
+
+    def graph_launch(self, stream=None):
+        check(lib().dspmm_batch_graph_launch(self.handle, stream), "batch_graph_launch")
+
+    def run_plain(self, stream=None):
+        check(lib().dspmm_batch_run_plain(self.handle, stream), "batch_run_plain")
+
+    def set_operands(self, index, B, C_out, stream=None):
+        check(lib().dspmm_batch_set_operands(self.handle, index, ptr(B), ptr(C_out), stream),
+              "batch_set_operands")
+
+    def benchmark(self, mode=BATCH_PERSISTENT, warmup=3, reps=20, stream=None):
+        mean = C.c_float()
+        arr = (C.c_float * reps)()
+        check(lib().dspmm_benchmark_batch(self.handle, mode, warmup, reps, C.byref(mean), arr,
+                                          stream), "benchmark_batch")
+        return mean.value, list(arr)
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.dspmm_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Turns rocprofv3 output of profile_workload.py into a short summary.
+
+Inputs: a kernel-trace results database (or kernel_stats CSV) and the two
+counter-collection CSVs (FETCH_SIZE, WRITE_SIZE).  Output: a JSON file with,
+per kernel, the average duration and the average FETCH/WRITE KiB per
+dispatch, plus the HBM traffic per dispatch scaled by the copy calibration
+(the device-to-device copy of known size in the same run), and a Markdown
+table beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import sqlite3
+from collections import defaultdict
+
+
+def short(name: str) -> str:
+    base = name.split("(")[0].replace("void ", "").strip()
+    return base.replace("dspmm::", "")
+
+
+def kernel_stats(path):
+    out = {}
+    if path.endswith(".db"):
+        con = sqlite3.connect(path)
+        for name, calls, avg in con.execute("select name, total_calls, average from top_kernels"):
+            out[name] = {"calls": calls, "avg_ns": avg}
+    else:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                out[r["Name"]] = {"calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"])}
+    return out
+
+
+def counters(path, counter):
+    """Per kernel: average counter value per dispatch; for the runtime's copy
+    kernel only the largest dispatch (the calibration copy) is kept."""
+    vals = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    out = {}
+    for k, v in vals.items():
+        out[k] = max(v) if "copyBuffer" in k else sum(v) / len(v)
+    return out
+
+
+def trace_only(args):
+    stats = kernel_stats(args.trace)
+    rows = sorted(stats.items(), key=lambda kv: -kv[1]["avg_ns"] * kv[1]["calls"])
+    with open(args.out, "w") as f:
+        f.write(f"# rocprofv3 kernel trace: {args.title}\n\n")
+        f.write(f"Command: `{args.command}`\n\n")
+        f.write("| kernel | calls | avg us | total us |\n|---|---|---|---|\n")
+        for name, st in rows[: args.top]:
+            f.write(f"| {short(name)} | {st['calls']} | {st['avg_ns'] / 1e3:.1f} | "
+                    f"{st['avg_ns'] * st['calls'] / 1e3:.0f} |\n")
+    print(json.dumps({"out": args.out}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True, help="kernel-trace .db or kernel_stats.csv")
+    ap.add_argument("--fetch", help="counter_collection.csv with FETCH_SIZE")
+    ap.add_argument("--write", help="counter_collection.csv with WRITE_SIZE")
+    ap.add_argument("--meta", help="JSON line printed by profile_workload.py")
+    ap.add_argument("--out", required=True, help="output .json (a .md is written beside it)")
+    ap.add_argument("--command", default="", help="command line that produced the inputs")
+    ap.add_argument("--title", default="", help="trace-only mode: title of the table")
+    ap.add_argument("--top", type=int, default=12)
+    args = ap.parse_args()
+    if not args.fetch:
+        return trace_only(args)
+
+    def one(pattern):
+        m = sorted(glob.glob(pattern, recursive=True))
+        if not m:
+            raise SystemExit(f"no file matches {pattern}")
+        return m[0]
+
+    meta = next(json.loads(l) for l in open(one(args.meta)) if l.startswith('{"workload"'))
+    stats = kernel_stats(one(args.trace))
+    fetch = counters(one(args.fetch), "FETCH_SIZE")
+    write = counters(one(args.write), "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k or "copy" in k.lower()]
+    if not copy:
+        raise SystemExit("calibration copy kernel not found in counters")
+    ck = copy[0]
+    cal_f = meta["copy_bytes"] / (fetch[ck] * 1024.0)
+    cal_w = meta["copy_bytes"] / (write[ck] * 1024.0)
+    kernels = {}
+    for name in sorted(set(fetch) | set(write)):
+        s = short(name)
+        st = next((v for k, v in stats.items() if short(k) == s), None)
+        f, w = fetch.get(name, 0.0), write.get(name, 0.0)
+        kernels[s] = {"full_name": name, "FETCH_SIZE_KiB": f, "WRITE_SIZE_KiB": w,
+                      "traffic_bytes": f * 1024 * cal_f + w * 1024 * cal_w,
+                      "avg_ns": st["avg_ns"] if st else None,
+                      "calls": st["calls"] if st else None}
+    cfg = meta["config"]
+    main_kernel = None
+    if cfg.get("variant") == "nnz_tiled":
+        tb = "unsigned short" if meta.get("bf16") else "float"
+        want = f"k_nnz_tiled<{tb}, {cfg['vec']}, {cfg['lanes']}, {4 if cfg['unroll'] <= 4 else 8}>"
+        main_kernel = want if want in kernels else None
+    res = {"meta": meta, "main_kernel": main_kernel,
+           "calibration": {"fetch": cal_f, "write": cal_w, "copy_kernel": ck},
+           "command": args.command, "kernels": kernels}
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+    md = os.path.splitext(args.out)[0] + ".md"
+    with open(md, "w") as f:
+        f.write(f"# rocprofv3 summary: {meta['workload']} N={meta['n']}"
+                f"{' BF16-B' if meta.get('bf16') else ''}\n\n")
+        f.write(f"Command: `{args.command}`\n\n")
+        f.write(f"Config: `{json.dumps(meta['config'])}`; algorithmic bytes per SpMM "
+                f"{meta['algorithmic_bytes']}\n\n")
+        f.write(f"Calibration (copy of {meta['copy_bytes']} B): fetch x{cal_f:.3f}, "
+                f"write x{cal_w:.3f}\n\n")
+        if main_kernel:
+            k = kernels[main_kernel]
+            f.write(f"Main kernel `{main_kernel}`: HBM traffic {k['traffic_bytes']:.4e} B per "
+                    f"launch = {k['traffic_bytes'] / meta['algorithmic_bytes']:.2f}x the "
+                    f"algorithmic bytes\n\n")
+        f.write("| kernel | calls | avg us | FETCH KiB | WRITE KiB | HBM bytes (calibrated) |\n")
+        f.write("|---|---|---|---|---|---|\n")
+        for s, k in kernels.items():
+            avg = f"{k['avg_ns'] / 1e3:.1f}" if k["avg_ns"] else "-"
+            f.write(f"| {s} | {k['calls']} | {avg} | {k['FETCH_SIZE_KiB']:.0f} | "
+                    f"{k['WRITE_SIZE_KiB']:.0f} | {k['traffic_bytes']:.3e} |\n")
+    print(json.dumps({"out": args.out, "kernels": list(kernels)}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+#!/usr/bin/env python3
+"""Runs one SpMM workload a few times, for use under rocprofv3.
+
+Before the SpMM steps it performs one device-to-device copy of a known size
+(256 MiB): its FETCH_SIZE / WRITE_SIZE readings calibrate the HBM byte
+counters (bytes moved / counter reading) for the kernels measured in the
+same run.
+
+  rocprofv3 --kernel-trace --stats -d OUT -o run -- \
+      python3 da-spmmul-kernel_amd/profile_workload.py --workload c2 --n 128
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o fetch -- python3 ...
+  rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT -o write -- python3 ...
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+COPY_BYTES = 256 << 20
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--bf16", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="0 = autotune")
+    ap.add_argument("--tile", type=int, default=0)
+    args = ap.parse_args()
+
+    import numpy as np
+
+    import dspmm as D
+    import workloads as W
+
+    a = {"c1": W.c1_matrix, "c2": W.c2_matrix, "c3": W.c3_matrix}[args.workload]()
+    nv = args.n
+    B = W.dense_b(2, a.nc, nv)
+    m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+    d_B = D.DeviceArray.from_numpy(B)
+    btype = D.DSPMM_R_32F
+    if args.bf16:
+        d_B = D.to_bf16_device(d_B)
+        btype = D.DSPMM_R_16BF
+    d_C = D.DeviceArray.zeros((a.nr, nv))
+
+    src = D.DeviceArray(COPY_BYTES, np.uint8)
+    dst = D.DeviceArray(COPY_BYTES, np.uint8)
+    D.check(D.lib().dspmm_memcpy_d2d(dst.ptr, src.ptr, COPY_BYTES, None), "d2d")
+    D.synchronize()
+
+    if args.variant:
+        cfg = D.make_config(args.variant, tile_nnz=args.tile)
+    else:
+        cfg, _ = m.autotune(d_B, d_C, nv, btype=btype, reps=3)
+    D.synchronize()
+    for _ in range(args.steps):
+        m.spmm(d_B, d_C, nv, btype=btype, config=cfg)
+    D.synchronize()
+    print(json.dumps({"workload": args.workload, "n": nv, "bf16": args.bf16, "nnz": a.nnz,
+                      "config": cfg.as_dict(), "copy_bytes": COPY_BYTES,
+                      "algorithmic_bytes": W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv,
+                                                               2 if args.bf16 else 4)}))
+
+
+if __name__ == "__main__":
+    main()

@@ -297,6 +297,60 @@ DSPMM_API dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout
 DSPMM_API dspmm_status_t dspmm_convert_f32_to_bf16(const float *src, void *dst, int64_t n,
                                                    dspmm_stream_t stream);
 
+
+/* ---------------------------------------------------------------------
+ * Batched SpMM (no reference counterpart): many independent row-major
+ * products C_i = A_i * B_i, each with its own matrix, nv and operands,
+ * in ONE persistent launch (plus one launch that combines split rows).
+ * Each item uses its matrix's autotuned (or heuristic) nnz-tiled config;
+ * results are bit-identical to dspmm_spmm_ex with that config.  Work is
+ * cut into wavefront units; each XCD owns a list of them, and a matrix is
+ * kept on one XCD when it is small enough for its B to stay in that XCD's
+ * L2.  The matrices must outlive the batch.
+ * ------------------------------------------------------------------- */
+typedef struct dspmm_batch_s *dspmm_batch_t;
+
+typedef struct dspmm_batch_item {
+    dspmm_matrix_t A;
+    int nv;
+    const void *B;   /* row-major, ldb >= nv; FP32 or BF16 per the batch */
+    int64_t ldb;
+    float *C;        /* row-major, ldc >= nv; overwritten */
+    int64_t ldc;
+} dspmm_batch_item;
+
+typedef struct dspmm_batch_info {
+    int count;
+    int units;           /* wavefront work units */
+    int segments;
+    int splits;          /* split rows combined after the main launch */
+    int blocks;          /* persistent workgroups */
+    int units_per_xcd[8];
+    size_t table_bytes;  /* device bytes of the schedule tables */
+    size_t carry_bytes;  /* device bytes of split-row partial sums */
+} dspmm_batch_info;
+
+/* blocks_per_cu: persistent workgroups per CU (0 = default). */
+DSPMM_API dspmm_status_t dspmm_batch_create(dspmm_batch_t *out, const dspmm_batch_item *items,
+                                            int count, dspmm_dtype_t btype, int blocks_per_cu);
+DSPMM_API dspmm_status_t dspmm_batch_destroy(dspmm_batch_t b);
+DSPMM_API dspmm_status_t dspmm_batch_describe(dspmm_batch_t b, dspmm_batch_info *info);
+/* Persistent launch (+ split combine) on `stream`. */
+DSPMM_API dspmm_status_t dspmm_batch_run(dspmm_batch_t b, dspmm_stream_t stream);
+/* Replays a hipGraph of dspmm_batch_run (captured on first call). */
+DSPMM_API dspmm_status_t dspmm_batch_graph_launch(dspmm_batch_t b, dspmm_stream_t stream);
+/* Baseline: one dspmm_spmm_ex launch per item, same configs, in order. */
+DSPMM_API dspmm_status_t dspmm_batch_run_plain(dspmm_batch_t b, dspmm_stream_t stream);
+/* Rebinds item `index` to new operands (same alignment class); the schedule
+ * and any captured graph stay valid.  Ordered on `stream`. */
+DSPMM_API dspmm_status_t dspmm_batch_set_operands(dspmm_batch_t b, int index, const void *B,
+                                                  float *C, dspmm_stream_t stream);
+/* mode 0: dspmm_batch_run, 1: dspmm_batch_graph_launch, 2: dspmm_batch_run_plain.
+ * Same timing contract as dspmm_benchmark_ex. */
+DSPMM_API dspmm_status_t dspmm_benchmark_batch(dspmm_batch_t b, int mode, int warmup, int reps,
+                                               float *ms_mean, float *ms_per_launch,
+                                               dspmm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,131 @@
+"""GPU tests of the batched path: one persistent launch over many matrices
+equals the single-matrix kernel bit for bit (same per-matrix config), is
+within the FP32 gate of the FP64 oracle, and its hipGraph replay follows
+operand rebinding (set_operands) across replays with eager work between
+them."""
+import numpy as np
+import pytest
+
+import dspmm as D
+import oracle as O
+import workloads as W
+from gpu_common import assert_bf16, assert_bitwise, assert_fp32, bits, note
+
+pytestmark = pytest.mark.gpu
+RM = D.DENSE_ROW_MAJOR
+
+
+class Item:
+    def __init__(self, i, a, nv, seed, bf16=False):
+        self.a, self.nv = a, nv
+        self.B = W.dense_b(seed, a.nc, nv)
+        self.m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+        self.d_B = D.DeviceArray.from_numpy(self.B)
+        if bf16:
+            self.d_B = D.to_bf16_device(self.d_B)
+        self.d_C = D.DeviceArray.from_numpy(np.full((a.nr, nv), np.nan, np.float32))
+        self.d_C_ref = D.DeviceArray.from_numpy(np.full((a.nr, nv), np.nan, np.float32))
+
+    def oracle(self, B=None):
+        B = self.B if B is None else B
+        c64, ab = O.spmm_fp64(RM, self.a.nr, self.a.nc, self.nv, self.a.rowPtr, self.a.colIdx,
+                              self.a.values, B)
+        rl = np.repeat(np.diff(self.a.rowPtr).astype(np.float64), self.nv)
+        return c64.reshape(-1), ab.reshape(-1), rl
+
+
+def make_items(count, seed=40, bf16=False, mmax=3000):
+    m, d, n = W.c4_params(count, seed)
+    items = []
+    for i in range(count):
+        mm = int(min(m[i], mmax))
+        a = W.gen_csr(W.GEN_BINOMIAL, mm, mm, seed * 1000 + i, p0=float(min(d[i], mm)))
+        items.append(Item(i, a, int(n[i]) if not bf16 else max(8, int(n[i]) // 2), seed + i, bf16))
+    # plus shapes the c4 draw rarely makes: a long split row, empty rows, nv=1/3
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 6, 900)
+    lens[[0, 1, 2, 899]] = 0
+    lens[450] = 2500
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(3000, int(k), replace=False)) for k in lens if k]).astype(np.int32)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    for nv in ((1, 3, 64) if not bf16 else (8, 64)):
+        items.append(Item(len(items), W.Csr(900, 3000, rp, ci, v), nv, seed + 77 + nv, bf16))
+    return items
+
+
+def test_batched_equals_single_and_oracle():
+    items = make_items(24)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    info = b.describe()
+    note(f"batch: {info}")
+    assert info["units"] > 0 and info["blocks"] % 8 == 0
+    b.run()
+    for it in items:                        # single-matrix path, same resolved config
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv)
+    D.synchronize()
+    for i, it in enumerate(items):
+        out, ref = it.d_C.numpy().reshape(-1), it.d_C_ref.numpy().reshape(-1)
+        assert_bitwise(out, ref, f"item {i} batched vs single")
+        c64, ab, rl = it.oracle()
+        assert_fp32(out, c64, ab, rl, f"item {i} (M={it.a.nr}, nv={it.nv})")
+
+
+def test_plain_launches_match_batched():
+    items = make_items(12, seed=41)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    b.run()
+    D.synchronize()
+    first = [it.d_C.numpy().copy() for it in items]
+    for it in items:
+        it.d_C.zero()
+    b.run_plain()
+    D.synchronize()
+    for i, it in enumerate(items):
+        assert np.array_equal(bits(first[i]), bits(it.d_C.numpy())), f"item {i}"
+
+
+def test_graph_replays_follow_rebound_operands():
+    items = make_items(10, seed=42)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    for rep in range(4):
+        # fresh B for every item each replay, bound without re-capturing
+        news = []
+        for i, it in enumerate(items):
+            Bn = W.dense_b(1000 + 10 * rep + i, it.a.nc, it.nv)
+            d_Bn = D.DeviceArray.from_numpy(Bn)
+            news.append((Bn, d_Bn))
+            b.set_operands(i, d_Bn, it.d_C)
+        b.graph_launch()
+        D.synchronize()
+        # eager work between replays
+        items[0].m.spmm(news[0][1], items[0].d_C_ref, items[0].nv)
+        D.synchronize()
+        assert np.array_equal(bits(items[0].d_C_ref.numpy()), bits(items[0].d_C.numpy()))
+        for i, it in enumerate(items):
+            c64, ab, rl = it.oracle(news[i][0])
+            assert_fp32(it.d_C.numpy().reshape(-1), c64, ab, rl, f"replay {rep} item {i}")
+
+
+def test_batched_bf16():
+    items = make_items(10, seed=43, bf16=True)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items], btype=D.DSPMM_R_16BF)
+    b.run()
+    D.synchronize()
+    for i, it in enumerate(items):
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, btype=D.DSPMM_R_16BF)
+    D.synchronize()
+    for i, it in enumerate(items):
+        out = it.d_C.numpy().reshape(-1)
+        assert_bitwise(out, it.d_C_ref.numpy().reshape(-1), f"bf16 item {i} batched vs single")
+        c64, ab, _ = it.oracle()
+        assert_bf16(out, c64, ab, f"bf16 item {i}")
+
+
+def test_benchmark_modes_run():
+    items = make_items(16, seed=44)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    for mode in (D.BATCH_PERSISTENT, D.BATCH_GRAPH, D.BATCH_PLAIN):
+        ms, per = b.benchmark(mode, warmup=1, reps=3)
+        note(f"mode {mode}: {ms * 1e3:.1f} us")
+        assert ms > 0 and len(per) == 3

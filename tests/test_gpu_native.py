@@ -61,12 +61,12 @@ def test_nnz_tiled_vector_widths_and_lanes(vec):
     c, m = matrix("zipf_mid")
     for nv in (4, 12, 128):
         for lanes in (0, 64):
-            cfg = D.make_config(D.VARIANT_NNZ_TILED, vec=vec, lanes=lanes, tile_nnz=128)
+            cfg = D.make_config(D.VARIANT_NNZ_TILED, vec=vec, lanes=lanes, tile_nnz=16)
             what = f"zipf_mid vec={vec} lanes={lanes} nv={nv}"
             note(what)
             out = run_native(c, m, nv, config=cfg)
             h32, c64, ab = c.ref(RM, nv)
-            split = np.repeat(split_rows(c, 128), nv)
+            split = np.repeat(split_rows(c, 16), nv)
             assert_bitwise(out[~split], h32[~split], what)
             assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), what, h32)
 
@@ -116,7 +116,7 @@ def test_bf16_dense_input(name):
     rounded = O.round_bf16(dense)
     h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, rounded).reshape(-1)
     c64, ab = O.spmm_fp64(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, dense)
-    for S in (64, 256):
+    for S in (32, 128):
         what = f"{name} bf16 S={S}"
         note(what)
         out = run_native(c, m, nv, config=D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=S),
