@@ -107,6 +107,64 @@ def bench_batched(D, W, np, count, reps=20):
     return res
 
 
+def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
+    """Config 5: a seeded Poisson stream (`rate` req/s) drawing 95% of the
+    requests from the c4 pool (`count` matrices, each with its N) and 5% from
+    `n_large` large power-law matrices (M=K=2e6, ~40M nnz, N=64).  B is
+    generated on the device per request (seeded by the request id).  The
+    same stream is replayed against (i) the continuous-batching server and
+    (ii) a plain server that launches each request on its own; then every
+    request is submitted at once to measure saturation throughput."""
+    t = time.perf_counter()
+    m, d, n = W.c4_params(count)
+    pool = [(W.c4_matrix(i, m[i], d[i]), int(n[i])) for i in range(count)]
+    large = [(W.c5_large_matrix(seed=500 + k), 64) for k in range(n_large)]
+    log(f"serving pool: {count} + {n_large} large (nnz {large[0][0].nnz if large else 0}) "
+        f"built in {time.perf_counter() - t:.1f}s")
+    at, cls = W.arrivals(n_req, rate, p_large)
+    rng = np.random.default_rng(55)
+    pick = [(count + int(rng.integers(max(1, n_large)))) if (c and n_large) else
+            int(rng.integers(count)) for c in cls]
+    allm = pool + large
+    res = {"requests": n_req, "rate_offered": rate, "large_fraction": float(np.mean(cls)),
+           "pool": count, "large_pool": n_large}
+    for plain in (False, True):
+        srv = D.Server(cache_bytes=200 << 30, operand_bytes=64 << 30, max_inflight=3,
+                       max_batch=64, plain=plain)
+        mids = [srv.add_matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values, preload=True)
+                for a, _ in allm]
+        warm = [D.Server.request(10_000_000 + i, mids[i], nv, i, discard=True)
+                for i, (_, nv) in enumerate(allm)]
+        srv.replay(warm, [0.0] * len(warm))
+        srv.poll()
+        key = "plain" if plain else "batched"
+        for mode, times in (("poisson", at), ("saturation", np.zeros(n_req))):
+            reqs = [D.Server.request(k, mids[pick[k]], allm[pick[k]][1], k, discard=True)
+                    for k in range(n_req)]
+            srv.replay(reqs, times)
+            comps = srv.poll()
+            lat = np.array([c["t_done"] - c["t_submit"] for c in comps])
+            big = np.array([pick[c["id"]] >= count for c in comps])
+            span = max(c["t_done"] for c in comps) - min(c["t_submit"] for c in comps)
+            ok = all(c["status"] == 0 for c in comps)
+            r = {"completed": len(comps), "all_ok": ok, "req_per_s": len(comps) / span,
+                 "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                 "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                 "mean_batch": float(np.mean([c["batch_size"] for c in comps]))}
+            if big.any() and (~big).any():
+                r["p50_ms_small"] = float(np.percentile(lat[~big], 50) * 1e3)
+                r["p99_ms_small"] = float(np.percentile(lat[~big], 99) * 1e3)
+                r["p50_ms_large"] = float(np.percentile(lat[big], 50) * 1e3)
+                r["p99_ms_large"] = float(np.percentile(lat[big], 99) * 1e3)
+            res[f"{key}_{mode}"] = r
+            log(f"serving {key} {mode}: {r}")
+        res[f"{key}_stats"] = {k: v for k, v in srv.stats().items()}
+        srv.close()
+    res["throughput_ratio_saturation"] = (res["batched_saturation"]["req_per_s"] /
+                                          res["plain_saturation"]["req_per_s"])
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +176,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip N=1/N=32/reference sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (c4) leg")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (c5) leg")
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-large", type=int, default=2, help="large matrices in the pool")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: the SpMM path here runs on a single GPU and "
@@ -186,6 +247,10 @@ def main():
     if not args.no_batch:
         extra["batched_c4"] = bench_batched(D, W, np, args.batch_count)
         log(f"batched: {extra['batched_c4']}")
+
+    if not args.no_serve:
+        extra["serving_c5"] = bench_serving(D, W, np, args.batch_count, args.serve_requests,
+                                            args.serve_large)
 
     # CPU baseline: the restated reference host loop, one thread, bounded sample.
     import importlib

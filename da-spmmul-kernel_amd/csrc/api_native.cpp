@@ -48,6 +48,39 @@ void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &t
     }
 }
 
+void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
+                      std::vector<int4> &tiles, std::vector<int4> &splits, int &nslots) {
+    tiles.clear();
+    splits.clear();
+    nslots = 0;
+    int r = 0;
+    while (r < nr) {
+        const int len = rp[(size_t)r + 1] - rp[(size_t)r];
+        if (len > row_max) {
+            if (len <= T) {
+                tiles.push_back(make_int4(r, r + 1, rp[(size_t)r], rp[(size_t)r + 1]));
+            } else {
+                const int nch = (len + T - 1) / T;
+                splits.push_back(make_int4(0, r, nslots, nch));
+                for (int c = 0; c < nch; c++) {
+                    const int pb = rp[(size_t)r] + c * T;
+                    tiles.push_back(make_int4(r, -1 - (nslots + c), pb, std::min(pb + T, rp[(size_t)r + 1])));
+                }
+                nslots += nch;
+            }
+            ++r;
+        } else {
+            const int rb = r, p0 = rp[(size_t)r];
+            int re = r + 1;
+            while (re < nr && re - rb < 256 && rp[(size_t)re + 1] - rp[(size_t)re] <= row_max &&
+                   rp[(size_t)re + 1] - p0 <= T)
+                ++re;
+            tiles.push_back(make_int4(rb, re, p0, rp[(size_t)re]));
+            r = re;
+        }
+    }
+}
+
 }  // namespace dspmm
 
 using namespace dspmm;
@@ -122,12 +155,14 @@ dspmm_status_t ensure_rowidx(dspmm_matrix_s *A) {
     return upload(&A->rowIdx, rows.data(), rows.size(), hipMemcpyHostToDevice, nullptr);
 }
 
-dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out) {
-    auto it = A->plans.find(S);
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind) {
+    const int key = 2 * S + kind;
+    auto it = A->plans.find(key);
     if (it != A->plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
     std::vector<int4> tiles, splits;
     int nslots = 0;
-    build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
+    if (kind == kPlanSpmv) build_spmv_tiles(A->h_rowPtr, A->nr, S, kSpmvRowMax, tiles, splits, nslots);
+    else build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
     NbPlan p;
     p.S = S;
     p.ntiles = (int)tiles.size();
@@ -144,7 +179,7 @@ dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out) {
     p.bytes = sizeof(int4) * (tiles.size() + splits.size());
     p.h_tiles.swap(tiles);
     p.h_splits.swap(splits);
-    auto res = A->plans.emplace(S, std::move(p));
+    auto res = A->plans.emplace(key, std::move(p));
     *out = &res.first->second;
     return DSPMM_STATUS_SUCCESS;
 }
@@ -167,11 +202,19 @@ void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *
     if (c.unroll <= 0) c.unroll = 8;
 }
 
-dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, dspmm_dtype_t bt) {
+constexpr int kSpmvTile = 2048;
+
+dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt) {
     (void)A;
     dspmm_config c{};
-    if (layout == DENSE_ROW_MAJOR || bt == DSPMM_R_16BF) c.variant = DSPMM_VARIANT_NNZ_TILED;
-    else c.variant = DSPMM_VARIANT_CSR_SCALAR;
+    if (nv == 1) {
+        c.variant = DSPMM_VARIANT_SPMV_TILED;
+        c.tile_nnz = kSpmvTile;
+    } else if (layout == DENSE_ROW_MAJOR || bt == DSPMM_R_16BF) {
+        c.variant = DSPMM_VARIANT_NNZ_TILED;
+    } else {
+        c.variant = DSPMM_VARIANT_CSR_SCALAR;
+    }
     return c;
 }
 
@@ -246,6 +289,8 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         d.cls = item_class(c.vec, c.lanes);
         d.ntiles = plan->ntiles;
         d.nslabs = div_up(nv, d.slab_w);
+        d.splits = plan->d_splits;
+        d.nsplits = plan->nsplits;
         a.ntiles = plan->ntiles;
         a.splits = plan->d_splits;
         a.nsplits = plan->nsplits;
@@ -256,6 +301,59 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         a.unroll = c.unroll;
         a.mid_event = mid;
         return launch_nnz_tiled(a, st);
+    }
+    case DSPMM_VARIANT_SPMV_TILED: {
+        if (nv != 1) return DSPMM_STATUS_NOT_SUPPORTED;
+        const int T = cfg.tile_nnz > 0 ? cfg.tile_nnz : kSpmvTile;
+        if (T < kSpmvRowMax || T > 16384) return DSPMM_STATUS_NOT_SUPPORTED;
+        NbPlan *plan = nullptr;
+        dspmm_status_t s = get_plan(A, T, &plan, kPlanSpmv);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        const int64_t ldcar = carry_ld(1);
+        const size_t need = sizeof(float) * (size_t)plan->nslots * (size_t)ldcar;
+        float *carry = nullptr;
+        if (need > 0) {
+            if (ws) {
+                if (ws_bytes < need) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
+                carry = reinterpret_cast<float *>(ws);
+            } else {
+                if (A->carry_bytes < need) {
+                    (void)hipFree(A->carry);
+                    A->carry = nullptr;
+                    A->carry_bytes = 0;
+                    s = hip_status(hipMalloc((void **)&A->carry, need));
+                    if (s != DSPMM_STATUS_SUCCESS) return s;
+                    A->carry_bytes = need;
+                }
+                carry = A->carry;
+            }
+        }
+        NbLaunch a;
+        ItemDesc &d = a.item;
+        d.tiles = plan->d_tiles;
+        d.rowPtr = A->rowPtr;
+        d.colIdx = A->colIdx;
+        d.values = A->values;
+        d.B = B;
+        d.C = C;
+        d.carry = carry;
+        d.ldb = layout == DENSE_ROW_MAJOR ? ldb : 1;   // stride of x
+        d.ldc = layout == DENSE_ROW_MAJOR ? ldc : 1;   // stride of y
+        d.ldcarry = ldcar;
+        d.nv = 1;
+        d.slab_w = 1;
+        d.S = T;
+        d.cls = 0;
+        d.ntiles = plan->ntiles;
+        d.nslabs = 1;
+        d.splits = plan->d_splits;
+        d.nsplits = plan->nsplits;
+        a.ntiles = plan->ntiles;
+        a.splits = plan->d_splits;
+        a.nsplits = plan->nsplits;
+        a.bf16 = bf16;
+        a.mid_event = mid;
+        return launch_spmv_tiled(a, st);
     }
     case DSPMM_VARIANT_CSR_SCALAR:
     case DSPMM_VARIANT_CSR_VECTOR:
@@ -308,7 +406,7 @@ dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_
     if (cfg && cfg->variant != DSPMM_VARIANT_AUTO) return *cfg;
     auto it = A->tuned.find(std::make_tuple((int)layout, nv, (int)bt));
     if (it != A->tuned.end()) return it->second;
-    return heuristic(A, layout, bt);
+    return heuristic(A, layout, nv, bt);
 }
 
 float median_of(std::vector<float> v) {
@@ -444,6 +542,13 @@ dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout
     std::lock_guard<std::mutex> lk(A->mu);
     dspmm_config c = resolve(A, layout, nv, btype, cfg);
     *bytes = 0;
+    if (c.variant == DSPMM_VARIANT_SPMV_TILED && nv == 1) {
+        NbPlan *plan = nullptr;
+        dspmm_status_t st = get_plan(A, c.tile_nnz > 0 ? c.tile_nnz : kSpmvTile, &plan, kPlanSpmv);
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(1);
+        return DSPMM_STATUS_SUCCESS;
+    }
     if (c.variant != DSPMM_VARIANT_NNZ_TILED) return DSPMM_STATUS_SUCCESS;
     // Tile size does not depend on pointers; use aligned placeholders.
     complete_nb(c, nv, nv, nv, reinterpret_cast<const void *>(256), reinterpret_cast<const void *>(256),
@@ -531,19 +636,20 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
     std::vector<dspmm_config> cand;
     const bool bf16 = btype == DSPMM_R_16BF;
     if (layout == DENSE_ROW_MAJOR) {
+        // Every feasible vector width, each with the lane count that covers
+        // nv in one slab; tile sizes around the default; both unrolls.
         dspmm_config base{};
         base.variant = DSPMM_VARIANT_NNZ_TILED;
         complete_nb(base, nv, ldb, ldc, B, C, bf16);
-        std::vector<int> vws{base.vec};
-        if (base.vec >= 4) vws.push_back(base.vec / 2);
-        for (int vw : vws) {
+        for (int vw = base.vec; vw >= 1; vw /= 2) {
             dspmm_config c{};
             c.variant = DSPMM_VARIANT_NNZ_TILED;
             c.vec = vw;
             complete_nb(c, nv, ldb, ldc, B, C, bf16);
+            if (c.vec != vw) continue;
             const int s0 = default_tile(c.lanes);
-            for (int S : {s0 / 4, s0 / 2, s0, s0 * 2}) {
-                if (S < 4 || S > 1024) continue;
+            for (int S : {s0 / 4, s0 / 2, s0, s0 * 2, s0 * 4}) {
+                if (S < 8 || S > 2048) continue;
                 if (nnz_tiled_lds_bytes(c.lanes, S) > kMaxTileLds) continue;
                 for (int u : {4, 8}) {
                     dspmm_config x = c;
@@ -552,6 +658,14 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
                     cand.push_back(x);
                 }
             }
+        }
+    }
+    if (nv == 1) {
+        for (int T : {1024, 2048, 4096}) {
+            dspmm_config c{};
+            c.variant = DSPMM_VARIANT_SPMV_TILED;
+            c.tile_nnz = T;
+            cand.push_back(c);
         }
     }
     if (!bf16) {
@@ -563,7 +677,7 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
         }
     }
     float best_t = 1e30f;
-    dspmm_config best_c = heuristic(A, layout, btype);
+    dspmm_config best_c = heuristic(A, layout, nv, btype);
     std::vector<float> ms((size_t)reps);
     for (const auto &c : cand) {
         float mean = 0;

@@ -27,6 +27,12 @@ constexpr size_t kMaxTileLds = 64 * 1024;
 
 void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &tiles,
                  std::vector<int4> &splits, int &nslots);
+// SpMV plan: tiles of <= T nonzeros and <= 256 rows, each row <= row_max
+// long; a longer row gets its own tile(s) of <= T nonzeros.
+void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
+                      std::vector<int4> &tiles, std::vector<int4> &splits, int &nslots);
+
+enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1 };
 
 }  // namespace dspmm
 
@@ -39,7 +45,7 @@ struct dspmm_matrix_s {
     double mean = 0, cv = 0;
     size_t base_bytes = 0;
     std::mutex mu;
-    std::map<int, dspmm::NbPlan> plans;                            // by tile size
+    std::map<int, dspmm::NbPlan> plans;                            // by 2 * tile size + kind
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     float *carry = nullptr;
     size_t carry_bytes = 0;
@@ -55,7 +61,7 @@ struct dspmm_matrix_s {
 namespace dspmm {
 
 // All of these expect the caller to hold A->mu.
-dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out);
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind = kPlanNnzTiled);
 int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,
                  bool bf16);

@@ -26,6 +26,8 @@ struct ItemDesc {
     int cls;               // 8 * log2(vw) + log2(lanes)
     int ntiles;
     int nslabs;
+    const int4 *splits;    // the plan's split records {0, row, first_slot, nchunks}
+    int nsplits;
 };
 
 // Arguments of one single-matrix nnz-tiled launch (plus its split combine).
@@ -49,17 +51,24 @@ struct BatchLaunch {
     const int *xoff = nullptr;         // 9 segment offsets + 8 unit counts
     int lds_ints_per_wave = 0;
     int nblocks = 0;
-    const int4 *splits = nullptr;      // {item, row, first_slot, nchunks}
-    int nsplits = 0;
+    const int *split_prefix = nullptr; // count+1 prefix sums of the items' nsplits
+    int count = 0;
+    int nsplits = 0;                   // total
     bool bf16 = false;
     hipEvent_t mid_event = nullptr;
 };
 
 dspmm_status_t launch_nnz_tiled(const NbLaunch &a, hipStream_t st);
 dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st);
+// SpMV (nv == 1) tiles: item.S is the tile size T, item.tiles a plan made
+// by build_spmv_tiles.
+dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st);
+constexpr int kSpmvRowMax = 256;   // longest row summed by one thread
 size_t nnz_tiled_lds_bytes(int lanes, int S);
 int item_class(int vw, int lanes);
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+// Seeded uniform [-1, 1) fill of n FP32 (or BF16) values.
+dspmm_status_t launch_fill_uniform(void *out, int64_t n, uint64_t seed, bool bf16, hipStream_t st);
 dspmm_status_t launch_f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t st);
 
 }  // namespace dspmm

@@ -260,50 +260,119 @@ __global__ __launch_bounds__(kBlock) void k_nnz_tiled_batched(const ItemDesc *__
 }
 
 // ---- split-row combine ------------------------------------------------------
-// One wavefront per split row.  For every column j the chunk partial sums
-// are added in one canonical order: lane l sums chunks l, l+64, l+128, ...
-// in sequence, then the 64 lane sums are combined by a butterfly (xor 1, 2,
-// 4, ..., 32; every lane ends with the same value).  Single-matrix and
-// batched launches use this same function, so their results agree bit for
-// bit.
-__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp) {
+// One workgroup per split row.  Canonical order, a function of (nv, nchunks)
+// only: Lc = min(256, next power of two >= nv) threads cover the columns,
+// the chunks are cut into P = 256 / Lc contiguous parts, each part is summed
+// in chunk order by one thread per column, and the P part sums are added in
+// part order.  Single-matrix and batched launches share this function, so
+// their results agree bit for bit.  A thread loads up to 16 chunk values
+// before adding them, so its loads overlap.
+__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp, float *lds) {
     const int row = sp.y, slot0 = sp.z, nch = sp.w;
-    const int ln = lane_id();
-    for (int j0 = 0; j0 < d.nv; j0 += 4) {
-        float v[4];
+    const int t = threadIdx.x;
+    int Lc = 1;
+    while (Lc < d.nv && Lc < kBlock) Lc <<= 1;
+    const int P = kBlock / Lc;
+    const int q = t / Lc, jl = t % Lc;
+    const int per = (nch + P - 1) / P;
+    const int c0 = min(nch, q * per), c1 = min(nch, c0 + per);
+    for (int jb = 0; jb < d.nv; jb += Lc) {          // uniform loop
+        const int j = jb + jl;
+        float s = 0.f;
+        if (j < d.nv) {
+            const float *src = d.carry + (int64_t)slot0 * d.ldcarry + j;
+            for (int c = c0; c < c1; c += 16) {
+                float v[16];
 #pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = 0.f;
-        for (int c = ln; c < nch; c += kWave) {
-            const float *src = d.carry + (int64_t)(slot0 + c) * d.ldcarry + j0;
+                for (int u = 0; u < 16; u++) v[u] = (c + u < c1) ? src[(int64_t)(c + u) * d.ldcarry] : 0.f;
 #pragma unroll
-            for (int k = 0; k < 4; k++) v[k] = v[k] + src[k];   // ldcarry pads to 8, j0+k < ldcarry
+                for (int u = 0; u < 16; u++)
+                    if (c + u < c1) s = s + v[u];
+            }
         }
-#pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = butterfly_sum<1, kWave>(v[k]);
-        if (ln < 4 && j0 + ln < d.nv) {
-            float o = v[0];
-            if (ln == 1) o = v[1];
-            if (ln == 2) o = v[2];
-            if (ln == 3) o = v[3];
-            d.C[(int64_t)row * d.ldc + j0 + ln] = o;
+        if (P > 1) {
+            lds[t] = s;
+            __syncthreads();
+            if (q == 0) {
+                for (int qq = 1; qq < P; qq++) s = s + lds[qq * Lc + jl];
+            }
+            __syncthreads();
         }
+        if (q == 0 && j < d.nv) d.C[(int64_t)row * d.ldc + j] = s;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_split_combine(ItemDesc d, const int4 *__restrict__ splits,
                                                           int nsplits) {
-    const int w = (int)(blockIdx.x * kBlock + threadIdx.x) / kWave;
-    if (w >= nsplits) return;                        // wave-uniform
-    combine_split(d, splits[w]);
+    __shared__ float lds[kBlock];
+    combine_split(d, splits[blockIdx.x], lds);
 }
 
+// Batched: workgroup b finds its item by binary search in the prefix sums
+// of the items' split counts.
 __global__ __launch_bounds__(kBlock) void k_split_combine_batched(const ItemDesc *__restrict__ items,
-                                                                  const int4 *__restrict__ splits,
-                                                                  int nsplits) {
-    const int w = (int)(blockIdx.x * kBlock + threadIdx.x) / kWave;
-    if (w >= nsplits) return;                        // wave-uniform
-    const int4 sp = splits[w];
-    combine_split(items[sp.x], sp);
+                                                                  const int *__restrict__ prefix,
+                                                                  int count) {
+    __shared__ float lds[kBlock];
+    const int b = blockIdx.x;
+    int lo = 0, hi = count - 1;                     // last item with prefix[i] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const ItemDesc &d = items[lo];
+    combine_split(d, d.splits[b - prefix[lo]], lds);
+}
+
+// ---- SpMV (nv == 1) -----------------------------------------------------------
+// One workgroup per tile of at most T nonzeros.  Phase 1 computes every
+// product a_p * x[col_p] into LDS with coalesced loads of (col, val) and
+// independent gathers of x.  Phase 2: in a tile of whole rows no longer
+// than kSpmvRowMax, thread i sums row r0+i from LDS in nonzero order --
+// bit-exact with the host loop; a tile holding (part of) one longer row is
+// reduced by the workgroup in a fixed tree order and written to y (one
+// chunk) or to its carry slot (several chunks, combined afterwards).
+template <typename TB>
+__global__ __launch_bounds__(kBlock) void k_spmv_tiled(ItemDesc d, int ntiles) {
+    extern __shared__ float prod[];
+    __shared__ float red[kBlock];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int4 tile = d.tiles[b];
+    const int pb = tile.z, pe = tile.w, n = pe - pb;
+    const int t = threadIdx.x;
+    const TB *__restrict__ x = reinterpret_cast<const TB *>(d.B);
+    for (int i = t; i < n; i += kBlock) {
+        const int c = ld_stream(d.colIdx + pb + i);
+        const float a = ld_stream(d.values + pb + i);
+        float xv[1];
+        load_b<TB, 1>(xv, x + (int64_t)c * d.ldb);
+        prod[i] = a * xv[0];
+    }
+    __syncthreads();
+    const bool tree = tile.y < 0 || (tile.y - tile.x == 1 && n > kSpmvRowMax);
+    if (!tree) {
+        const int r = tile.x + t;
+        if (r < tile.y) {
+            const int s0 = d.rowPtr[r] - pb, s1 = d.rowPtr[r + 1] - pb;
+            float s = 0.f;
+            for (int p = s0; p < s1; p++) s = s + prod[p];
+            d.C[(int64_t)r * d.ldc] = s;
+        }
+        return;
+    }
+    // fixed-order tree: thread t sums t, t+256, ... then pairwise halving
+    float s = 0.f;
+    for (int i = t; i < n; i += kBlock) s = s + prod[i];
+    red[t] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (t < h) red[t] = red[t] + red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (tile.y < 0) d.carry[(int64_t)(-1 - tile.y) * d.ldcarry] = red[0];
+        else d.C[(int64_t)tile.x * d.ldc] = red[0];
+    }
 }
 
 __global__ void k_fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {
@@ -313,6 +382,32 @@ __global__ void k_fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {
 __global__ void k_fill_zero_u8(uint8_t *__restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// Seeded U[lo, hi) fill (SplitMix64 of seed ^ index, 24-bit mantissa);
+// BF16 output is the RNE rounding of the same FP32 value.
+__device__ __forceinline__ float seeded_uniform(uint64_t seed, int64_t i, float lo, float hi) {
+    uint64_t z = seed ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull);
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u = (float)(z >> 40) * 0x1.0p-24f;
+    return lo + (hi - lo) * u;
+}
+__device__ __forceinline__ uint16_t to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) u |= 0x00400000u;
+    else u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__global__ void k_fill_uniform(void *__restrict__ out, int64_t n, uint64_t seed, int bf16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float v = seeded_uniform(seed, i, -1.f, 1.f);
+        if (bf16) reinterpret_cast<uint16_t *>(out)[i] = to_bf16_rne(v);
+        else reinterpret_cast<float *>(out)[i] = v;
+    }
 }
 
 // FP32 -> BF16, round to nearest even on the bit pattern; NaNs stay NaN.
@@ -392,8 +487,8 @@ dspmm_status_t launch_nnz_tiled(const NbLaunch &a, hipStream_t st) {
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine, dim3(div_up(a.nsplits, kBlock / kWave)), dim3(kBlock), 0,
-                           st, a.item, a.splits, a.nsplits);
+        hipLaunchKernelGGL(k_split_combine, dim3(a.nsplits), dim3(kBlock), 0, st, a.item, a.splits,
+                           a.nsplits);
         s = launch_status();
     }
     return s;
@@ -413,11 +508,29 @@ dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st) {
     }
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine_batched, dim3(div_up(a.nsplits, kBlock / kWave)),
-                           dim3(kBlock), 0, st, a.items, a.splits, a.nsplits);
+        hipLaunchKernelGGL(k_split_combine_batched, dim3(a.nsplits), dim3(kBlock), 0, st, a.items,
+                           a.split_prefix, a.count);
         return launch_status();
     }
     return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
+    if (a.ntiles == 0) return DSPMM_STATUS_SUCCESS;
+    const size_t lds = sizeof(float) * (size_t)a.item.S;
+    if (a.bf16)
+        hipLaunchKernelGGL(k_spmv_tiled<uint16_t>, dim3(a.ntiles), dim3(kBlock), lds, st, a.item, a.ntiles);
+    else
+        hipLaunchKernelGGL(k_spmv_tiled<float>, dim3(a.ntiles), dim3(kBlock), lds, st, a.item, a.ntiles);
+    dspmm_status_t s = launch_status();
+    if (s != DSPMM_STATUS_SUCCESS) return s;
+    if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
+    if (a.nsplits > 0) {
+        hipLaunchKernelGGL(k_split_combine, dim3(a.nsplits), dim3(kBlock), 0, st, a.item, a.splits,
+                           a.nsplits);
+        s = launch_status();
+    }
+    return s;
 }
 
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st) {
@@ -432,6 +545,13 @@ dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st) {
         hipLaunchKernelGGL(k_fill_zero_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                            reinterpret_cast<uint8_t *>(p), n);
     }
+    return launch_status();
+}
+
+dspmm_status_t launch_fill_uniform(void *out, int64_t n, uint64_t seed, bool bf16, hipStream_t st) {
+    if (n == 0) return DSPMM_STATUS_SUCCESS;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_fill_uniform, dim3(blocks), dim3(256), 0, st, out, n, seed, bf16 ? 1 : 0);
     return launch_status();
 }
 

@@ -44,8 +44,9 @@ CSV_NAMES = {(ALG_CSR_SCALAR, 0): "SR_RB_RM", (ALG_CSR_VECTOR, 0): "PR_RB_RM",
 DSPMM_R_32F, DSPMM_R_16BF = 0, 1
 VARIANT_AUTO, VARIANT_NNZ_TILED = 0, 1
 VARIANT_CSR_SCALAR, VARIANT_CSR_VECTOR, VARIANT_COO_SCALAR, VARIANT_COO_VECTOR = 2, 3, 4, 5
+VARIANT_SPMV_TILED = 6
 VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
-                 4: "coo_scalar", 5: "coo_vector"}
+                 4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled"}
 MATRIX_WITH_COO = 0x1
 
 STATUS_SUCCESS = 0
@@ -99,6 +100,33 @@ class BatchInfo(C.Structure):
     _fields_ = [("count", C.c_int), ("units", C.c_int), ("segments", C.c_int),
                 ("splits", C.c_int), ("blocks", C.c_int), ("units_per_xcd", C.c_int * 8),
                 ("table_bytes", C.c_size_t), ("carry_bytes", C.c_size_t)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("cache_bytes", C.c_size_t), ("operand_bytes", C.c_size_t),
+                ("max_inflight", C.c_int), ("max_batch", C.c_int), ("plain", C.c_int),
+                ("btype", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("matrix", C.c_int), ("nv", C.c_int), ("seed", C.c_uint64),
+                ("B", C.c_void_p), ("C", C.c_void_p), ("flags", C.c_uint)]
+
+
+REQUEST_DISCARD_RESULT = 0x1
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("batch_size", C.c_int),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("C", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("batches", C.c_int64), ("cache_hits", C.c_int64), ("cache_misses", C.c_int64),
+                ("evictions", C.c_int64), ("cache_used", C.c_size_t), ("cache_peak", C.c_size_t),
+                ("operand_peak", C.c_size_t), ("mean_batch", C.c_double)]
 
 
 _lib = None
@@ -164,6 +192,16 @@ def _declare(lib):
         "dspmm_batch_set_operands": ([_P, _I, _P, _P, _P], _ST),
         "dspmm_benchmark_batch": ([_P, _I, _I, _I, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    _P], _ST),
+        "dspmm_server_now": ([], C.c_double),
+        "dspmm_server_create": ([C.POINTER(C.c_void_p), C.POINTER(ServerConfig)], _ST),
+        "dspmm_server_destroy": ([_P], _ST),
+        "dspmm_server_add_matrix": ([_P, _I, _I, _I, _P, _P, _P, _I, C.POINTER(C.c_int)], _ST),
+        "dspmm_server_submit": ([_P, C.POINTER(Request)], _ST),
+        "dspmm_server_replay": ([_P, C.POINTER(Request), C.POINTER(C.c_double), _I], _ST),
+        "dspmm_server_poll": ([_P, C.POINTER(Completion), _I], _I),
+        "dspmm_server_release": ([_P, C.c_uint64], _ST),
+        "dspmm_server_drain": ([_P], _ST),
+        "dspmm_server_get_stats": ([_P, C.POINTER(ServerStats)], _ST),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -495,3 +533,99 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+STATUS_ADMISSION_REJECTED = 6
+
+
+class Server:
+    """The serving loop (dspmm_server_*): resident CSR cache, HBM-budgeted
+    admission and continuous batching, run by a library thread."""
+
+    def __init__(self, cache_bytes=200 << 30, operand_bytes=16 << 30, max_inflight=3,
+                 max_batch=128, plain=False, btype=DSPMM_R_32F):
+        cfg = ServerConfig(cache_bytes, operand_bytes, max_inflight, max_batch, int(plain), btype)
+        h = C.c_void_p()
+        check(lib().dspmm_server_create(C.byref(h), C.byref(cfg)), "server_create")
+        self.handle = h.value
+
+    def add_matrix(self, nr, nc, rowPtr, colIdx, values, preload=False) -> int:
+        rowPtr = np.ascontiguousarray(rowPtr, np.int32)
+        colIdx = np.ascontiguousarray(colIdx, np.int32)
+        values = np.ascontiguousarray(values, np.float32)
+        mid = C.c_int()
+        check(lib().dspmm_server_add_matrix(self.handle, nr, nc, int(colIdx.size),
+                                            rowPtr.ctypes.data, colIdx.ctypes.data,
+                                            values.ctypes.data, int(preload), C.byref(mid)),
+              "server_add_matrix")
+        return mid.value
+
+    @staticmethod
+    def request(rid, matrix, nv, seed, B=None, C_out=None, discard=False) -> Request:
+        return Request(rid, matrix, nv, seed, ptr(B), ptr(C_out),
+                       REQUEST_DISCARD_RESULT if discard else 0)
+
+    def submit(self, req: Request) -> int:
+        return lib().dspmm_server_submit(self.handle, C.byref(req))
+
+    def replay(self, reqs, at):
+        n = len(reqs)
+        arr = (Request * max(1, n))(*reqs)
+        t = np.ascontiguousarray(at, np.float64)
+        check(lib().dspmm_server_replay(self.handle, arr, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                        n), "server_replay")
+
+    def drain(self):
+        check(lib().dspmm_server_drain(self.handle), "server_drain")
+
+    def poll(self, max_n=4096):
+        out = (Completion * max_n)()
+        got = []
+        while True:
+            n = lib().dspmm_server_poll(self.handle, out, max_n)
+            got.extend({"id": c.id, "status": c.status, "batch_size": c.batch_size,
+                        "t_submit": c.t_submit, "t_admit": c.t_admit, "t_done": c.t_done,
+                        "C": c.C} for c in out[:n])
+            if n < max_n:
+                return got
+
+    def release(self, rid):
+        check(lib().dspmm_server_release(self.handle, rid), "server_release")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().dspmm_server_get_stats(self.handle, C.byref(st)), "server_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.dspmm_server_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def seeded_uniform(seed, n):
+    """Host copy of the server's device generator of B (SplitMix64 of
+    seed ^ i * golden, 24-bit uniform, mapped to [-1, 1))."""
+    with np.errstate(over="ignore"):
+        i = np.arange(int(n), dtype=np.uint64)
+        z = np.uint64(seed) ^ (i * np.uint64(0x9E3779B97F4A7C15))
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+    return (np.float32(-1.0) + np.float32(2.0) * u).astype(np.float32)
+
+
+def device_read(ptr_value, shape, dtype=np.float32):
+    """Copies a device buffer (raw address) to a new numpy array."""
+    out = np.empty(shape, dtype)
+    if out.nbytes:
+        check(lib().dspmm_memcpy_d2h(out.ctypes.data, ptr_value, out.nbytes), "memcpy_d2h")
+    return out

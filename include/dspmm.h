@@ -242,7 +242,10 @@ typedef enum dspmm_variant {
     DSPMM_VARIANT_CSR_SCALAR = 2,
     DSPMM_VARIANT_CSR_VECTOR = 3,
     DSPMM_VARIANT_COO_SCALAR = 4,
-    DSPMM_VARIANT_COO_VECTOR = 5
+    DSPMM_VARIANT_COO_VECTOR = 5,
+    DSPMM_VARIANT_SPMV_TILED = 6 /* nv == 1 only: workgroup per nnz tile, products
+                                    in LDS, per-row sums (row-exact), tree for
+                                    long rows; tile_nnz = tile size */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {
@@ -330,7 +333,9 @@ typedef struct dspmm_batch_info {
     size_t carry_bytes;  /* device bytes of split-row partial sums */
 } dspmm_batch_info;
 
-/* blocks_per_cu: persistent workgroups per CU (0 = default). */
+/* blocks_per_cu > 0: a persistent grid of that many workgroups per CU that
+ * walk the per-XCD work lists; 0: one workgroup per four work units (the
+ * same schedule, spread over as many workgroups as it needs). */
 DSPMM_API dspmm_status_t dspmm_batch_create(dspmm_batch_t *out, const dspmm_batch_item *items,
                                             int count, dspmm_dtype_t btype, int blocks_per_cu);
 DSPMM_API dspmm_status_t dspmm_batch_destroy(dspmm_batch_t b);
@@ -350,6 +355,84 @@ DSPMM_API dspmm_status_t dspmm_batch_set_operands(dspmm_batch_t b, int index, co
 DSPMM_API dspmm_status_t dspmm_benchmark_batch(dspmm_batch_t b, int mode, int warmup, int reps,
                                                float *ms_mean, float *ms_per_launch,
                                                dspmm_stream_t stream);
+
+
+/* ---------------------------------------------------------------------
+ * Serving loop (no reference counterpart).  A server thread keeps CSR
+ * matrices resident in HBM (an LRU cache bounded by cache_bytes), admits
+ * queued requests while their dense operands fit an HBM arena of
+ * operand_bytes, and runs continuous batching: whenever one of
+ * max_inflight stream slots is free, every admissible queued request (up
+ * to max_batch) is launched as one batched SpMM on that slot, so batches
+ * overlap on the device and requests that arrive meanwhile join the next
+ * one.  Results are bit-identical to running the request alone (the
+ * kernels are deterministic and each matrix keeps its config).  In plain
+ * mode every request is one dspmm_spmm_ex launch on a single stream (the
+ * baseline).  Times are seconds on dspmm_server_now()'s steady clock.
+ * ------------------------------------------------------------------- */
+typedef struct dspmm_server_s *dspmm_server_t;
+
+typedef struct dspmm_server_config {
+    size_t cache_bytes;     /* HBM budget of resident matrices (CSR + plans) */
+    size_t operand_bytes;   /* HBM arena for server-owned B, C and split carries */
+    int max_inflight;       /* batches (slots / streams) in flight, >= 1 */
+    int max_batch;          /* requests per batch, >= 1 */
+    int plain;              /* 1: one launch per request, no batching */
+    dspmm_dtype_t btype;    /* storage of server-generated B (FP32 or BF16) */
+} dspmm_server_config;
+
+typedef struct dspmm_request {
+    uint64_t id;
+    int matrix;         /* id from dspmm_server_add_matrix */
+    int nv;             /* dense columns */
+    uint64_t seed;      /* B is generated on the device from this seed (U[-1,1),
+                           row-major, ld = nv) unless B is given */
+    const void *B;      /* optional caller-owned device B (row-major, ld = nv) */
+    float *C;           /* optional caller-owned device C (row-major, ld = nv);
+                           NULL: server-owned, valid until dspmm_server_release */
+    unsigned flags;     /* DSPMM_REQUEST_* */
+} dspmm_request;
+
+/* Server-owned C is freed as soon as the request completes (the result is
+ * not kept; for load generators and benchmarks). */
+#define DSPMM_REQUEST_DISCARD_RESULT 0x1u
+
+typedef struct dspmm_completion {
+    uint64_t id;
+    int status;             /* dspmm_status_t */
+    int batch_size;
+    double t_submit, t_admit, t_done;
+    const float *C;         /* device result */
+} dspmm_completion;
+
+typedef struct dspmm_server_stats {
+    int64_t submitted, completed, rejected, batches;
+    int64_t cache_hits, cache_misses, evictions;
+    size_t cache_used, cache_peak, operand_peak;
+    double mean_batch;
+} dspmm_server_stats;
+
+DSPMM_API double dspmm_server_now(void);
+DSPMM_API dspmm_status_t dspmm_server_create(dspmm_server_t *s, const dspmm_server_config *cfg);
+DSPMM_API dspmm_status_t dspmm_server_destroy(dspmm_server_t s);
+/* Registers a CSR matrix (host arrays are copied); it is uploaded when first
+ * needed, or now with `preload`. */
+DSPMM_API dspmm_status_t dspmm_server_add_matrix(dspmm_server_t s, int nr, int nc, int nnz,
+                                                 const int *h_rowPtr, const int *h_colIdx,
+                                                 const float *h_values, int preload,
+                                                 int *matrix_id);
+/* Non-blocking.  DSPMM_STATUS_ADMISSION_REJECTED if the request can never
+ * fit (its matrix exceeds cache_bytes or its operands exceed operand_bytes). */
+DSPMM_API dspmm_status_t dspmm_server_submit(dspmm_server_t s, const dspmm_request *req);
+/* Submits req[i] at time t0 + at[i] (at[] ascending, seconds), from the
+ * calling thread, then waits until all of them have completed. */
+DSPMM_API dspmm_status_t dspmm_server_replay(dspmm_server_t s, const dspmm_request *req,
+                                             const double *at, int n);
+/* Completions not yet returned; returns how many were written (<= max). */
+DSPMM_API int dspmm_server_poll(dspmm_server_t s, dspmm_completion *out, int max);
+DSPMM_API dspmm_status_t dspmm_server_release(dspmm_server_t s, uint64_t id);
+DSPMM_API dspmm_status_t dspmm_server_drain(dspmm_server_t s);
+DSPMM_API dspmm_status_t dspmm_server_get_stats(dspmm_server_t s, dspmm_server_stats *st);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,19 @@
+// Compiles and links a C++ caller of the C-ABI and the SpmvSpmmProblem
+// mirror (include/spmvspmm.hpp); runs only calls that need no GPU.
+#include <cstdio>
+#include <cstring>
+
+#include "spmvspmm.hpp"
+
+int main() {
+    // Host-side argument checks return before touching the device.
+    SpmvSpmmProblem prob(4, 4, 0, 8, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    prob.run(DENSE_ROW_MAJOR, 3, ALG_CSR_SCALAR);               // prints, returns
+    if (prob.benchmark(DENSE_ROW_MAJOR, 6, ALG_CSR_SCALAR) != -1.0f) return 1;
+    if (dspmm_spmm(ALG_COO_VECTOR, DENSE_COL_MAJOR, 4, 4, 0, 5, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr, nullptr) != DSPMM_STATUS_NOT_SUPPORTED)
+        return 2;
+    if (std::strcmp(dspmm_get_error_string(DSPMM_STATUS_SUCCESS), "success") != 0) return 3;
+    std::printf("ok %s\n", dspmm_version());
+    return 0;
+}

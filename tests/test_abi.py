@@ -111,3 +111,20 @@ def test_generators_are_deterministic():
         assert np.all(np.diff(seg) > 0) and (seg.size == 0 or (seg[0] >= 0 and seg[-1] < 3000))
     s = W.c3_matrix(8)
     assert s.nnz == (3 * 8 - 2) ** 3
+
+
+def test_cxx_caller_compiles_links_and_runs(tmp_path):
+    """A C++ program using include/dspmm.h + include/spmvspmm.hpp builds with
+    g++ against libdspmm.so and runs its GPU-free calls."""
+    import shutil
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    src = os.path.join(os.path.dirname(__file__), "cxx", "abi_link.cpp")
+    exe = str(tmp_path / "abi_link")
+    libdir = os.path.dirname(D.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{D.INCLUDE_DIR}", src, "-o", exe,
+                    f"-L{libdir}", "-ldspmm", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.startswith("ok dspmm")
+    assert out.stderr.count("nv must be power of 2") == 2

@@ -61,8 +61,11 @@ def test_batched_equals_single_and_oracle():
     note(f"batch: {info}")
     assert info["units"] > 0 and info["blocks"] % 8 == 0
     b.run()
-    for it in items:                        # single-matrix path, same resolved config
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv)
+    for it in items:
+        # single-matrix path with the config the batch resolved: the matrix's
+        # own nnz-tiled config (nv == 1 would otherwise pick the SpMV kernel)
+        cfg = D.make_config(D.VARIANT_NNZ_TILED) if it.nv == 1 else None
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=cfg)
     D.synchronize()
     for i, it in enumerate(items):
         out, ref = it.d_C.numpy().reshape(-1), it.d_C_ref.numpy().reshape(-1)

@@ -189,3 +189,42 @@ def test_matrix_from_device_arrays_and_describe():
     assert abs(d.mean_row_len - c.rowlen.mean()) < 1e-9
     assert d.hbm_bytes >= 4 * (c.nr + 1 + 2 * c.nnz)
     D.lib().dspmm_matrix_destroy(h.value)
+
+
+@pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20", "c2"])
+@pytest.mark.parametrize("T", [1024, 2048, 4096])
+def test_spmv_tiled(name, T):
+    """SpMV kernel: rows of <= 256 nonzeros are summed like the host loop (bit
+    for bit); longer rows are tree-reduced (and combined across tiles)."""
+    c, m = matrix(name)
+    h32, c64, ab = c.ref(RM, 1)
+    cfg = D.make_config(D.VARIANT_SPMV_TILED, tile_nnz=T)
+    for layout in (RM, CM):
+        what = f"{name} spmv T={T} layout={layout}"
+        note(what)
+        out = run_native(c, m, 1, layout=layout, config=cfg)
+        short = c.rowlen <= 256
+        assert_bitwise(out[short], h32[short], what + " (rows <= 256)")
+        assert_fp32(out, c64, ab, c.rowlen, what, h32)
+
+
+def test_spmv_strided_and_bf16():
+    c, m = matrix("zipf_mid")
+    ld = 5
+    X = W.gen_dense(91, c.nc * ld).reshape(c.nc, ld)
+    d_X = D.DeviceArray.from_numpy(X)
+    d_Y = D.DeviceArray.from_numpy(np.full((c.nr, 3), -1.0, np.float32))
+    m.spmm(d_X, d_Y, 1, ldb=ld, ldc=3, config=D.make_config(D.VARIANT_SPMV_TILED))
+    D.synchronize()
+    y = d_Y.numpy()
+    assert np.all(y[:, 1:] == -1.0)
+    x0 = np.ascontiguousarray(X[:, 0])
+    h32 = O.spmm_host32(RM, c.nr, c.nc, 1, c.rowPtr, c.colIdx, c.values, x0).reshape(-1)
+    short = c.rowlen <= 256
+    assert_bitwise(y[:, 0][short], h32[short], "strided spmv")
+    d16 = D.to_bf16_device(c.d_B)
+    out = run_native(c, m, 1, config=D.make_config(D.VARIANT_SPMV_TILED), btype=D.DSPMM_R_16BF,
+                     B=d16)
+    xr = O.round_bf16(c.dense(1))
+    h32r = O.spmm_host32(RM, c.nr, c.nc, 1, c.rowPtr, c.colIdx, c.values, xr).reshape(-1)
+    assert_bitwise(out[short], h32r[short], "bf16 spmv")

@@ -1,0 +1,134 @@
+"""GPU tests of the serving loop: continuous batching returns, for every
+request, exactly the result of the same request run alone; results match
+the FP64 oracle on the device-generated B; a small cache evicts and stays
+correct; oversized requests are rejected; plain mode gives the same bits."""
+import numpy as np
+import pytest
+
+import dspmm as D
+import oracle as O
+import workloads as W
+from gpu_common import assert_fp32, bits, note
+
+pytestmark = pytest.mark.gpu
+RM = D.DENSE_ROW_MAJOR
+
+
+def make_pool(n=8, seed=60, mmax=2500):
+    m, d, nvs = W.c4_params(n, seed)
+    pool = []
+    for i in range(n):
+        mm = int(min(m[i], mmax))
+        pool.append((W.gen_csr(W.GEN_BINOMIAL, mm, mm, seed * 1000 + i, p0=float(min(d[i], mm))),
+                     int(nvs[i])))
+    # one power-law matrix with split rows
+    pool.append((W.gen_csr(W.GEN_ZIPF, 6000, 6000, seed + 7, 1.8, 1, 3000, W.VAL_NORMAL), 64))
+    return pool
+
+
+def register(srv, pool, preload=True):
+    return [srv.add_matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values, preload=preload)
+            for a, _ in pool]
+
+
+def results(srv, comps, pool, mids_of):
+    out = {}
+    for c in comps:
+        assert c["status"] == 0, c
+        a, nv = pool[mids_of[c["id"]]]
+        out[c["id"]] = D.device_read(c["C"], (a.nr * nv,)).copy()
+        srv.release(c["id"])
+    return out
+
+
+def test_server_alone_equals_in_the_mix():
+    pool = make_pool()
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16)
+    mids = register(srv, pool)
+    probe = [(1000 + k, k % len(pool), 777 + k) for k in range(len(pool))]
+    alone = {}
+    for rid, mi, seed in probe:                         # one at a time, server otherwise idle
+        assert srv.submit(srv.request(rid, mids[mi], pool[mi][1], seed)) == 0
+        srv.drain()
+        comps = srv.poll()
+        assert len(comps) == 1 and comps[0]["batch_size"] == 1
+        alone.update(results(srv, comps, pool, {rid: mi}))
+    # the same requests (same matrix, nv, seed) mixed into a burst of 120 others
+    rng = np.random.default_rng(3)
+    reqs, at, mi_of = [], [], {}
+    for k in range(120):
+        mi = int(rng.integers(len(pool)))
+        reqs.append(srv.request(k, mids[mi], pool[mi][1], 5000 + k))
+        at.append(k * 20e-6)
+        mi_of[k] = mi
+    for j, (rid, mi, seed) in enumerate(probe):
+        reqs.insert(10 * j + 5, srv.request(2000 + rid, mids[mi], pool[mi][1], seed))
+        at.insert(10 * j + 5, at[10 * j + 4])
+        mi_of[2000 + rid] = mi
+    srv.replay(reqs, at)
+    comps = srv.poll()
+    assert len(comps) == len(reqs)
+    sizes = [c["batch_size"] for c in comps]
+    note(f"batch sizes: max {max(sizes)}, mean {np.mean(sizes):.1f}; stats {srv.stats()}")
+    assert max(sizes) > 1, "requests arriving together must share batches"
+    got = results(srv, comps, pool, mi_of)
+    for rid, mi, seed in probe:
+        assert np.array_equal(bits(got[2000 + rid]), bits(alone[rid])), f"request {rid}"
+    # accuracy of a few against the oracle on the same device-generated B
+    for rid, mi, seed in probe[:4]:
+        a, nv = pool[mi]
+        B = D.seeded_uniform(seed, a.nc * nv)
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        assert_fp32(alone[rid], c64.reshape(-1), ab.reshape(-1), rl, f"served request {rid}")
+    srv.close()
+
+
+def test_server_small_cache_evicts_and_stays_correct():
+    pool = make_pool(6, seed=61)
+    need = max(4 * (a.nr + 1) + 9 * a.nnz + a.nr for a, _ in pool) + (1 << 20)
+    srv = D.Server(cache_bytes=2 * need + (1 << 20), operand_bytes=512 << 20, max_inflight=2,
+                   max_batch=8)
+    mids = register(srv, pool, preload=False)
+    reqs = [srv.request(k, mids[k % len(pool)], pool[k % len(pool)][1], 90 + k)
+            for k in range(3 * len(pool))]
+    srv.replay(reqs, [0.0] * len(reqs))
+    comps = srv.poll()
+    st = srv.stats()
+    note(f"stats {st}")
+    assert st["evictions"] > 0 and st["completed"] == len(reqs)
+    got = results(srv, comps, pool, {k: k % len(pool) for k in range(len(reqs))})
+    for k in range(len(pool)):                          # the same seed always gives one answer
+        a, nv = pool[k]
+        B = D.seeded_uniform(90 + k, a.nc * nv)
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"evicting request {k}", h32)
+    srv.close()
+
+
+def test_server_rejects_what_never_fits():
+    pool = make_pool(2, seed=62)
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=4096, max_inflight=1, max_batch=4)
+    mids = register(srv, pool)
+    st = srv.submit(srv.request(1, mids[0], pool[0][1], 1))
+    assert st == D.STATUS_ADMISSION_REJECTED
+    assert srv.stats()["rejected"] == 1
+    srv.close()
+
+
+def test_plain_mode_same_bits_as_batched():
+    pool = make_pool(5, seed=63)
+    outs = []
+    for plain in (False, True):
+        srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16,
+                       plain=plain)
+        mids = register(srv, pool)
+        reqs = [srv.request(k, mids[k % len(pool)], pool[k % len(pool)][1], 300 + k)
+                for k in range(2 * len(pool))]
+        srv.replay(reqs, [0.0] * len(reqs))
+        outs.append(results(srv, srv.poll(), pool, {k: k % len(pool) for k in range(len(reqs))}))
+        srv.close()
+    for k in outs[0]:
+        assert np.array_equal(bits(outs[0][k]), bits(outs[1][k])), f"request {k}"
