@@ -50,6 +50,7 @@ VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
 MATRIX_WITH_COO = 0x1
 
 STATUS_SUCCESS = 0
+STATUS_INVALID_VALUE = 1
 STATUS_NOT_SUPPORTED = 2
 
 # The 12 reference-named launchers, in spmv.h / spmm.h order.
@@ -120,6 +121,12 @@ class Completion(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("batch_size", C.c_int),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
                 ("C", C.c_void_p)]
+
+
+class Coo(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nc", C.c_int), ("nnz", C.c_int),
+                ("rowIdx", C.POINTER(C.c_int)), ("colIdx", C.POINTER(C.c_int)),
+                ("values", C.POINTER(C.c_float))]
 
 
 class ServerStats(C.Structure):
@@ -202,6 +209,9 @@ def _declare(lib):
         "dspmm_server_release": ([_P, C.c_uint64], _ST),
         "dspmm_server_drain": ([_P], _ST),
         "dspmm_server_get_stats": ([_P, C.POINTER(ServerStats)], _ST),
+        "dspmm_mtx_read": ([C.c_char_p, C.c_uint, C.POINTER(Coo)], _ST),
+        "dspmm_coo_free": ([C.POINTER(Coo)], None),
+        "dspmm_csr_rowptr": ([_I, _I, _P, _P], _ST),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -607,6 +617,36 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+MTX_KEEP_DIAGONAL = 0x1
+MTX_ALIGN_VALUES = 0x2
+
+
+def read_mtx(path, flags=0):
+    """Reads a coordinate Matrix-Market file with the library's loader
+    (readMtx<float>, util/spmatrix.hpp:286-333).  Returns
+    (nr, nc, rowIdx, colIdx, values), sorted by (row, column)."""
+    coo = Coo()
+    check(lib().dspmm_mtx_read(os.fsencode(path), int(flags), C.byref(coo)), f"mtx_read {path}")
+    try:
+        n = coo.nnz
+        r = np.ctypeslib.as_array(coo.rowIdx, (max(n, 1),))[:n].copy()
+        c = np.ctypeslib.as_array(coo.colIdx, (max(n, 1),))[:n].copy()
+        v = np.ctypeslib.as_array(coo.values, (max(n, 1),))[:n].copy()
+        return coo.nr, coo.nc, r, c, v
+    finally:
+        lib().dspmm_coo_free(C.byref(coo))
+
+
+def csr_rowptr(nr, rowIdx):
+    """rowPtr[nr + 1] from sorted row indices (generateCsrRowPtrArray,
+    spmvspmm_benchmark.cu:41-76)."""
+    rows = np.ascontiguousarray(rowIdx, dtype=np.int32)
+    out = np.empty(int(nr) + 1, np.int32)
+    check(lib().dspmm_csr_rowptr(int(nr), int(rows.size), rows.ctypes.data if rows.size else None,
+                                 out.ctypes.data), "csr_rowptr")
+    return out
 
 
 def seeded_uniform(seed, n):

@@ -434,6 +434,36 @@ DSPMM_API dspmm_status_t dspmm_server_release(dspmm_server_t s, uint64_t id);
 DSPMM_API dspmm_status_t dspmm_server_drain(dspmm_server_t s);
 DSPMM_API dspmm_status_t dspmm_server_get_stats(dspmm_server_t s, dspmm_server_stats *st);
 
+/* ---- Matrix-Market input (host) ------------------------------------------- */
+
+/* A host COO matrix sorted by (row, column); arrays are malloc'ed by the
+ * library and released with dspmm_coo_free. */
+typedef struct dspmm_coo {
+    int nr, nc, nnz;
+    int *rowIdx, *colIdx;
+    float *values;
+} dspmm_coo;
+
+#define DSPMM_MTX_KEEP_DIAGONAL 0x1u  /* symmetric files: keep diagonal entries */
+#define DSPMM_MTX_ALIGN_VALUES  0x2u  /* symmetric files: values follow their entries */
+
+/* Reads a coordinate Matrix-Market file (real, integer or pattern; general or
+ * symmetric; 1-based indices).  Replaces readMtx<int,float>
+ * (util/spmatrix.hpp:286-333) with mm_read_banner / mm_read_mtx_crd_size
+ * (util/mmio.hpp:215-336), called at spmvspmm_benchmark.cu:150.  With
+ * flags == 0 the result equals the reference's for the same file: symmetric
+ * files are mirrored, duplicates removed, diagonal entries dropped, and the
+ * compaction after a dropped entry moves indices but not values
+ * (util/spmatrix.hpp:264-283); the two flags above change the last two.
+ * Complex files: DSPMM_STATUS_NOT_SUPPORTED; malformed files, dense arrays
+ * and out-of-range indices: DSPMM_STATUS_INVALID_VALUE. */
+DSPMM_API dspmm_status_t dspmm_mtx_read(const char *path, unsigned flags, dspmm_coo *out);
+DSPMM_API void dspmm_coo_free(dspmm_coo *coo);
+/* rowPtr[nr + 1] from row indices sorted ascending; replaces
+ * generateCsrRowPtrArray (spmvspmm_benchmark.cu:41-76).  Unsorted or
+ * out-of-range rows: DSPMM_STATUS_INVALID_VALUE. */
+DSPMM_API dspmm_status_t dspmm_csr_rowptr(int nr, int nnz, const int *rowIdx, int *rowPtr);
+
 #ifdef __cplusplus
 }
 #endif
