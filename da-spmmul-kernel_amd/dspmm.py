@@ -27,6 +27,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdspmm.so")
 GEN_PATH = os.path.join(_HERE, "lib", "libdspmm_gen.so")
+ROCSPARSE_LIB_PATH = os.path.join(_HERE, "lib", "libdspmm_rocsparse.so")
+HARNESS_PATH = os.path.join(_HERE, "bin", "spmvspmm")
 INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
 
 # Reference enums (spmvspmm.h:5-10, helper.h:256-264).
@@ -388,6 +390,14 @@ def make_config(variant=VARIANT_AUTO, vec=0, lanes=0, tile_nnz=0, unroll=0) -> C
     return Config(variant, vec, lanes, tile_nnz, unroll)
 
 
+def config_from_dict(d) -> Config:
+    """Inverse of Config.as_dict."""
+    names = {v: k for k, v in VARIANT_NAMES.items()}
+    v = d["variant"]
+    return make_config(names.get(v, v) if isinstance(v, str) else int(v), int(d.get("vec", 0)),
+                       int(d.get("lanes", 0)), int(d.get("tile_nnz", 0)), int(d.get("unroll", 0)))
+
+
 class Matrix:
     """Device-resident CSR matrix (native API)."""
 
@@ -647,6 +657,85 @@ def csr_rowptr(nr, rowIdx):
     check(lib().dspmm_csr_rowptr(int(nr), int(rows.size), rows.ctypes.data if rows.size else None,
                                  out.ctypes.data), "csr_rowptr")
     return out
+
+
+_rs_lib = None
+
+
+def rocsparse_lib():
+    """The loaded rocSPARSE comparator library (lib/libdspmm_rocsparse.so)."""
+    global _rs_lib
+    if _rs_lib is None:
+        lib()  # status strings and the device helpers
+        if not os.path.exists(ROCSPARSE_LIB_PATH):
+            raise RuntimeError(f"{ROCSPARSE_LIB_PATH} is missing: build it first")
+        r = C.CDLL(ROCSPARSE_LIB_PATH)
+        sig = {
+            "dspmm_rocsparse_create": ([C.POINTER(C.c_void_p), _I, _I, _I, _P, _P, _P, _P, _P, _P,
+                                        _P], _ST),
+            "dspmm_rocsparse_destroy": ([_P], _ST),
+            "dspmm_rocsparse_columns": ([_I], _I),
+            "dspmm_rocsparse_column_name": ([_I, _I], C.c_char_p),
+            "dspmm_rocsparse_column_layout": ([_I, _I], _I),
+            "dspmm_rocsparse_run": ([_P, _I, _I], _ST),
+            "dspmm_rocsparse_benchmark": ([_P, _I, _I, _I, _I, C.POINTER(C.c_float)], _ST),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(r, name)
+            f.argtypes, f.restype = args, res
+        _rs_lib = r
+    return _rs_lib
+
+
+class RocSparse:
+    """rocSPARSE generic SpMV/SpMM over the caller's device arrays: the
+    comparator columns of the sweep (CusparseSpmmProblem,
+    cusparse_helper.h:136-470).  Dense operands as the reference lays them
+    out: row-major ld = nv, column-major ld = nc (B) / nr (C)."""
+
+    def __init__(self, nr, nc, nnz, rowPtr, rowIdx, colIdx, values, dnInput, dnOutput,
+                 stream=None):
+        self.rs = rocsparse_lib()
+        self.h = C.c_void_p()
+        check(self.rs.dspmm_rocsparse_create(C.byref(self.h), nr, nc, nnz, ptr(rowPtr),
+                                             ptr(rowIdx), ptr(colIdx), ptr(values), ptr(dnInput),
+                                             ptr(dnOutput), stream), "rocsparse_create")
+
+    def columns(self, nv):
+        return self.rs.dspmm_rocsparse_columns(nv)
+
+    def name(self, nv, c):
+        return self.rs.dspmm_rocsparse_column_name(nv, c).decode()
+
+    def layout(self, nv, c):
+        return self.rs.dspmm_rocsparse_column_layout(nv, c)
+
+    def run(self, nv, c):
+        """Runs column c; False when rocSPARSE does not support it."""
+        st = self.rs.dspmm_rocsparse_run(self.h, nv, c)
+        if st == STATUS_NOT_SUPPORTED:
+            return False
+        check(st, f"rocsparse run {self.name(nv, c)}")
+        return True
+
+    def benchmark(self, nv, c, warmup=20, reps=200):
+        ms = C.c_float()
+        st = self.rs.dspmm_rocsparse_benchmark(self.h, nv, c, warmup, reps, C.byref(ms))
+        if st == STATUS_NOT_SUPPORTED:
+            return None
+        check(st, f"rocsparse benchmark {self.name(nv, c)}")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.rs.dspmm_rocsparse_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def seeded_uniform(seed, n):

@@ -52,12 +52,36 @@ def counters(path, counter):
     return out
 
 
+def last_dispatches(trace_csv, prefix, n):
+    """Average duration (ns) of the last n dispatches whose kernel name
+    (without 'void ' and namespaces) starts with prefix."""
+    durs = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            if short(r["Kernel_Name"]).startswith(prefix):
+                durs.append((int(r["Start_Timestamp"]),
+                             int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                             short(r["Kernel_Name"])))
+    durs.sort()
+    tail = durs[-n:]
+    if not tail:
+        return None
+    return {"kernel": tail[-1][2], "dispatches": len(tail),
+            "avg_ns": sum(d for _, d, _ in tail) / len(tail)}
+
+
 def trace_only(args):
     stats = kernel_stats(args.trace)
     rows = sorted(stats.items(), key=lambda kv: -kv[1]["avg_ns"] * kv[1]["calls"])
     with open(args.out, "w") as f:
         f.write(f"# rocprofv3 kernel trace: {args.title}\n\n")
         f.write(f"Command: `{args.command}`\n\n")
+        if args.last and args.kernel:
+            trace_csv = args.trace.replace("kernel_stats.csv", "kernel_trace.csv")
+            t = last_dispatches(trace_csv, args.kernel, args.last)
+            if t:
+                f.write(f"Timed region: the last {t['dispatches']} dispatches of `{t['kernel']}` "
+                        f"average {t['avg_ns'] / 1e3:.1f} us\n\n")
         f.write("| kernel | calls | avg us | total us |\n|---|---|---|---|\n")
         for name, st in rows[: args.top]:
             f.write(f"| {short(name)} | {st['calls']} | {st['avg_ns'] / 1e3:.1f} | "
@@ -75,6 +99,9 @@ def main():
     ap.add_argument("--command", default="", help="command line that produced the inputs")
     ap.add_argument("--title", default="", help="trace-only mode: title of the table")
     ap.add_argument("--top", type=int, default=12)
+    ap.add_argument("--kernel", default="", help="trace-only mode: kernel name prefix whose "
+                    "last --last dispatches are averaged (the timed steps)")
+    ap.add_argument("--last", type=int, default=0)
     args = ap.parse_args()
     if not args.fetch:
         return trace_only(args)

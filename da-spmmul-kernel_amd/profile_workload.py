@@ -7,7 +7,8 @@ counters (bytes moved / counter reading) for the kernels measured in the
 same run.
 
   rocprofv3 --kernel-trace --stats -d OUT -o run -- \
-      python3 da-spmmul-kernel_amd/profile_workload.py --workload c2 --n 128
+      python3 da-spmmul-kernel_amd/profile_workload.py --workload c2 --n 128 \
+      --variant 1 --vec 1 --lanes 64 --tile 128 --unroll 8
   rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o fetch -- python3 ...
   rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT -o write -- python3 ...
 """
@@ -31,7 +32,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--bf16", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="0 = autotune")
+    ap.add_argument("--vec", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--unroll", type=int, default=0)
     args = ap.parse_args()
 
     import numpy as np
@@ -56,9 +60,10 @@ def main():
     D.synchronize()
 
     if args.variant:
-        cfg = D.make_config(args.variant, tile_nnz=args.tile)
+        cfg = D.make_config(args.variant, args.vec, args.lanes, args.tile, args.unroll)
     else:
         cfg, _ = m.autotune(d_B, d_C, nv, btype=btype, reps=3)
+    cfg = m.resolve(nv, btype=btype, config=cfg)
     D.synchronize()
     for _ in range(args.steps):
         m.spmm(d_B, d_C, nv, btype=btype, config=cfg)

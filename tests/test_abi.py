@@ -14,21 +14,54 @@ import dspmm as D
 HEADER = os.path.join(D.INCLUDE_DIR, "dspmm.h")
 
 
+RS_HEADER = os.path.join(D.INCLUDE_DIR, "dspmm_rocsparse.h")
+LIBS = [(D.LIB_PATH, HEADER), (D.ROCSPARSE_LIB_PATH, RS_HEADER)]
+
+
 def test_library_exports_every_declared_symbol(dlib):
-    names = D.declared_symbols()
+    names = D.declared_symbols(HEADER)
     assert len(names) >= 40
     missing = [n for n in names if not hasattr(dlib, n)]
     assert not missing, missing
 
 
-def test_exported_symbols_are_exactly_the_declared_ones():
-    out = subprocess.run(["nm", "-D", "--defined-only", D.LIB_PATH], capture_output=True,
+def test_every_header_belongs_to_a_library():
+    import glob
+    assert sorted(glob.glob(os.path.join(D.INCLUDE_DIR, "*.h"))) == sorted(h for _, h in LIBS)
+
+
+@pytest.mark.parametrize("lib,header", LIBS, ids=["dspmm", "rocsparse_comparator"])
+def test_exported_symbols_are_exactly_the_declared_ones(lib, header):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True,
                          text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    declared = set(D.declared_symbols())
-    assert declared <= exported
+    declared = set(D.declared_symbols(header))
+    assert declared and declared <= exported
     extra = {s for s in exported - declared if not s.startswith("_")}
     assert not extra, f"undeclared exports: {sorted(extra)}"
+
+
+def test_comparator_loads_and_names_the_reference_columns():
+    rs = D.rocsparse_lib()
+    assert rs.dspmm_rocsparse_columns(1) == 4 and rs.dspmm_rocsparse_columns(8) == 10
+    assert rs.dspmm_rocsparse_columns(0) == 0
+    names = [rs.dspmm_rocsparse_column_name(8, c).decode() for c in range(10)]
+    assert len(set(names)) == 10 and all(names)
+    lay = [rs.dspmm_rocsparse_column_layout(8, c) for c in range(10)]
+    assert lay == [D.DENSE_ROW_MAJOR] * 4 + [D.DENSE_COL_MAJOR] * 6
+    assert rs.dspmm_rocsparse_column_layout(8, 10) == -1
+    h = C.c_void_p()
+    assert rs.dspmm_rocsparse_create(C.byref(h), 4, 4, 1, None, None, None, None, None, None,
+                                     None) == D.STATUS_INVALID_VALUE
+
+
+def test_harness_usage_and_bad_file(tmp_path):
+    exe = D.HARNESS_PATH
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 2 and "usage: spmvspmm" in out.stderr
+    bad = os.path.join(os.path.dirname(__file__), "golden", "mtx", "bad_banner.mtx")
+    out = subprocess.run([exe, bad], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 2 and "Could not read" in out.stderr
 
 
 def test_reference_launcher_names_present():
