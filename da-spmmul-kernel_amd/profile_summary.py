@@ -137,6 +137,11 @@ def main():
         tb = "unsigned short" if meta.get("bf16") else "float"
         want = f"k_nnz_tiled<{tb}, {cfg['vec']}, {cfg['lanes']}, {4 if cfg['unroll'] <= 4 else 8}>"
         main_kernel = want if want in kernels else None
+    if main_kernel is None:
+        # otherwise the kernel with the most time per call, runtime copies and fills aside
+        cand = {k: v for k, v in kernels.items()
+                if v["avg_ns"] and "copyBuffer" not in k and "fill" not in k}
+        main_kernel = max(cand, key=lambda k: cand[k]["avg_ns"]) if cand else None
     res = {"meta": meta, "main_kernel": main_kernel,
            "calibration": {"fetch": cal_f, "write": cal_w, "copy_kernel": ck},
            "command": args.command, "kernels": kernels}
