@@ -33,7 +33,9 @@ dspmm_status_t build_schedule(const dspmm_batch_item *items, int count, dspmm_dt
             (A->nnz > 0 && !it.B))
             return DSPMM_STATUS_INVALID_VALUE;
         std::lock_guard<std::mutex> lk(A->mu);
-        dspmm_config c = resolve(A, DENSE_ROW_MAJOR, it.nv, btype, nullptr);
+        // tuned configuration if cached, else the static default: the batch
+        // schedule's efficiency is not what single launches were measured on
+        dspmm_config c = resolve(A, DENSE_ROW_MAJOR, it.nv, btype, nullptr, false);
         if (c.variant != DSPMM_VARIANT_NNZ_TILED) {
             c = dspmm_config{};
             c.variant = DSPMM_VARIANT_NNZ_TILED;
@@ -196,6 +198,20 @@ dspmm_status_t run_batched(dspmm_batch_s *b, hipStream_t st, hipEvent_t mid) {
     return launch_nnz_tiled_batched(a, st);
 }
 
+// One launch per item with the item's own AUTO configuration (its tuned
+// configuration when cached, else the selector's, else the static default)
+// and the matrix's own workspace.
+dspmm_status_t run_plain_own(dspmm_batch_s *b, hipStream_t st) {
+    for (int i = 0; i < b->count; i++) {
+        const dspmm_batch_item &it = b->items[(size_t)i];
+        dspmm_status_t s = dspmm_spmm_ex(it.A, DENSE_ROW_MAJOR, it.nv, it.B, it.ldb, b->btype, it.C,
+                                         it.ldc, nullptr, nullptr, 0,
+                                         reinterpret_cast<dspmm_stream_t>(st));
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+    }
+    return DSPMM_STATUS_SUCCESS;
+}
+
 dspmm_status_t run_plain(dspmm_batch_s *b, hipStream_t st) {
     for (int i = 0; i < b->count; i++) {
         const dspmm_batch_item &it = b->items[(size_t)i];
@@ -323,11 +339,12 @@ dspmm_status_t dspmm_batch_set_operands(dspmm_batch_t b, int index, const void *
 
 dspmm_status_t dspmm_benchmark_batch(dspmm_batch_t b, int mode, int warmup, int reps,
                                      float *ms_mean, float *ms_per_launch, dspmm_stream_t stream) {
-    if (!b || reps < 1 || warmup < 0 || mode < 0 || mode > 2) return DSPMM_STATUS_INVALID_VALUE;
+    if (!b || reps < 1 || warmup < 0 || mode < 0 || mode > 3) return DSPMM_STATUS_INVALID_VALUE;
     const hipStream_t hs = as_hip(stream);
     auto once = [&]() -> dspmm_status_t {
         if (mode == 0) return run_batched(b, hs, nullptr);
         if (mode == 1) return dspmm_batch_graph_launch(b, stream);
+        if (mode == 3) return run_plain_own(b, hs);
         return run_plain(b, hs);
     };
     dspmm_status_t st = DSPMM_STATUS_SUCCESS;

@@ -53,12 +53,17 @@ void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
     tiles.clear();
     splits.clear();
     nslots = 0;
+    // Rows longer than row_max but at most kSpmvLongTiles * T get a tile of
+    // their own; they are listed first, so the longest work starts first.
+    std::vector<int4> longs;
     int r = 0;
     while (r < nr) {
         const int len = rp[(size_t)r + 1] - rp[(size_t)r];
         if (len > row_max) {
             if (len <= T) {
                 tiles.push_back(make_int4(r, r + 1, rp[(size_t)r], rp[(size_t)r + 1]));
+            } else if (len <= kSpmvLongTiles * T) {
+                longs.push_back(make_int4(r, r + 1, rp[(size_t)r], rp[(size_t)r + 1]));
             } else {
                 const int nch = (len + T - 1) / T;
                 splits.push_back(make_int4(0, r, nslots, nch));
@@ -79,6 +84,63 @@ void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
             r = re;
         }
     }
+    tiles.insert(tiles.begin(), longs.begin(), longs.end());
+}
+
+void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, int nr, int nc,
+                        int umax, int smax, std::vector<int4> &tiles, std::vector<int> &ucols,
+                        std::vector<uint16_t> &lidx, double &reuse) {
+    tiles.clear();
+    ucols.clear();
+    lidx.assign(ci.size(), 0);
+    std::vector<int> in_tile((size_t)nc, -1), pos((size_t)nc, 0), in_row((size_t)nc, -1);
+    int cur = -1, cur_u = 0, cur_nnz = 0, cur_rows = 0;
+    int64_t direct_nnz = 0;
+    auto fresh_cols = [&](int r, int tile) {           // distinct columns of r not in `tile`
+        int f = 0;
+        for (int p = rp[(size_t)r]; p < rp[(size_t)r + 1]; p++) {
+            const int c = ci[(size_t)p];
+            if (in_tile[(size_t)c] != tile && in_row[(size_t)c] != r) {
+                in_row[(size_t)c] = r;
+                f++;
+            }
+        }
+        for (int p = rp[(size_t)r]; p < rp[(size_t)r + 1]; p++) in_row[(size_t)ci[(size_t)p]] = -1;
+        return f;
+    };
+    for (int r = 0; r < nr; r++) {
+        const int len = rp[(size_t)r + 1] - rp[(size_t)r];
+        int f = cur >= 0 ? fresh_cols(r, cur) : 0;
+        if (cur < 0 || cur_u + f > umax || cur_nnz + len > smax || cur_rows == kCachedRowsMax) {
+            cur = -1;
+            f = fresh_cols(r, -2);
+            if (f > umax || len > smax) {               // does not fit alone: direct tile
+                tiles.push_back(make_int4(r, -1, rp[(size_t)r], (int)ucols.size()));
+                direct_nnz += len;
+                continue;
+            }
+            cur = (int)tiles.size();
+            tiles.push_back(make_int4(r, r, rp[(size_t)r], (int)ucols.size()));
+            cur_u = cur_nnz = cur_rows = 0;
+        }
+        const int u0 = tiles[(size_t)cur].w;
+        for (int p = rp[(size_t)r]; p < rp[(size_t)r + 1]; p++) {
+            const int c = ci[(size_t)p];
+            if (in_tile[(size_t)c] != cur) {
+                in_tile[(size_t)c] = cur;
+                pos[(size_t)c] = (int)ucols.size() - u0;
+                ucols.push_back(c);
+            }
+            lidx[(size_t)p] = (uint16_t)pos[(size_t)c];
+        }
+        cur_u += f;
+        cur_nnz += len;
+        cur_rows++;
+        tiles[(size_t)cur].y = r + 1;
+    }
+    tiles.push_back(make_int4(nr, nr, ci.empty() ? 0 : (int)ci.size(), (int)ucols.size()));
+    const double reads = (double)ucols.size() + (double)direct_nnz;
+    reuse = reads > 0 ? (double)ci.size() / reads : 1.0;
 }
 
 }  // namespace dspmm
@@ -96,6 +158,8 @@ void free_matrix(dspmm_matrix_s *A) {
     for (auto &kv : A->plans) {
         hipFree(kv.second.d_tiles);
         hipFree(kv.second.d_splits);
+        hipFree(kv.second.d_ucols);
+        hipFree(kv.second.d_lidx);
     }
     hipFree(A->carry);
     delete A;
@@ -155,28 +219,85 @@ dspmm_status_t ensure_rowidx(dspmm_matrix_s *A) {
     return upload(&A->rowIdx, rows.data(), rows.size(), hipMemcpyHostToDevice, nullptr);
 }
 
+int cached_smax(int umax) { return std::min(4096, 4 * umax); }
+
+// Host copy of the column indices (device -> host, for plans that need them).
+dspmm_status_t host_colidx(const dspmm_matrix_s *A, std::vector<int> &ci) {
+    ci.resize((size_t)A->nnz);
+    if (A->nnz == 0) return DSPMM_STATUS_SUCCESS;
+    return hip_status(hipMemcpy(ci.data(), A->colIdx, sizeof(int) * (size_t)A->nnz,
+                                hipMemcpyDeviceToHost));
+}
+
+double column_reuse(dspmm_matrix_s *A) {
+    if (A->reuse16 >= 0) return A->reuse16;
+    A->reuse16 = 1.0;
+    if (A->nnz == 0 || A->nr == 0) return A->reuse16;
+    // up to 256 windows of 16 consecutive rows, spread over the matrix
+    const int win = 16, nwin = std::min(256, std::max(1, A->nr / win));
+    std::vector<int> buf;
+    std::vector<int> cols;
+    int64_t nz = 0, distinct = 0;
+    for (int w = 0; w < nwin; w++) {
+        const int r0 = (int)((int64_t)w * (A->nr - std::min(win, A->nr)) / std::max(1, nwin - 1));
+        const int r1 = std::min(A->nr, r0 + win);
+        const int p0 = A->h_rowPtr[(size_t)r0], p1 = A->h_rowPtr[(size_t)r1];
+        if (p1 <= p0) continue;
+        buf.resize((size_t)(p1 - p0));
+        if (hipMemcpy(buf.data(), A->colIdx + p0, sizeof(int) * buf.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return A->reuse16;
+        cols.assign(buf.begin(), buf.end());
+        std::sort(cols.begin(), cols.end());
+        nz += (int64_t)cols.size();
+        distinct += (int64_t)(std::unique(cols.begin(), cols.end()) - cols.begin());
+    }
+    if (distinct > 0) A->reuse16 = (double)nz / (double)distinct;
+    return A->reuse16;
+}
+
 dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind) {
-    const int key = 2 * S + kind;
+    const int key = 4 * S + kind;
     auto it = A->plans.find(key);
     if (it != A->plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
     std::vector<int4> tiles, splits;
+    std::vector<int> ucols;
+    std::vector<uint16_t> lidx;
     int nslots = 0;
-    if (kind == kPlanSpmv) build_spmv_tiles(A->h_rowPtr, A->nr, S, kSpmvRowMax, tiles, splits, nslots);
-    else build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
+    double reuse = 0;
+    if (kind == kPlanSpmv) {
+        build_spmv_tiles(A->h_rowPtr, A->nr, S, kSpmvRowMax, tiles, splits, nslots);
+    } else if (kind == kPlanCached) {
+        std::vector<int> ci;
+        dspmm_status_t s = host_colidx(A, ci);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        build_cached_tiles(A->h_rowPtr, ci, A->nr, A->nc, S, cached_smax(S), tiles, ucols, lidx,
+                           reuse);
+    } else {
+        build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
+    }
     NbPlan p;
     p.S = S;
-    p.ntiles = (int)tiles.size();
+    p.ntiles = (int)tiles.size() - (kind == kPlanCached ? 1 : 0);
     p.nsplits = (int)splits.size();
     p.nslots = nslots;
+    p.reuse = reuse;
     dspmm_status_t st = upload(&p.d_tiles, tiles.data(), tiles.size(), hipMemcpyHostToDevice, nullptr);
     if (st == DSPMM_STATUS_SUCCESS)
         st = upload(&p.d_splits, splits.data(), splits.size(), hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = upload(&p.d_ucols, ucols.data(), ucols.size(), hipMemcpyHostToDevice, nullptr);
+    if (st == DSPMM_STATUS_SUCCESS)
+        st = upload(&p.d_lidx, lidx.data(), lidx.size(), hipMemcpyHostToDevice, nullptr);
     if (st != DSPMM_STATUS_SUCCESS) {
         hipFree(p.d_tiles);
         hipFree(p.d_splits);
+        hipFree(p.d_ucols);
+        hipFree(p.d_lidx);
         return st;
     }
-    p.bytes = sizeof(int4) * (tiles.size() + splits.size());
+    p.bytes = sizeof(int4) * (tiles.size() + splits.size()) + sizeof(int) * ucols.size() +
+              sizeof(uint16_t) * lidx.size();
     p.h_tiles.swap(tiles);
     p.h_splits.swap(splits);
     auto res = A->plans.emplace(key, std::move(p));
@@ -202,7 +323,9 @@ void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *
     if (c.unroll <= 0) c.unroll = 8;
 }
 
-constexpr int kSpmvTile = 2048;
+constexpr int kSpmvTile = 1024;
+constexpr int kCachedUmaxDefault = 128;
+constexpr double kCachedMinReuse = 1.5;   // sampled nonzeros per distinct column
 
 dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt) {
     (void)A;
@@ -302,10 +425,56 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         a.mid_event = mid;
         return launch_nnz_tiled(a, st);
     }
+    case DSPMM_VARIANT_NNZ_CACHED: {
+        if (layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
+        dspmm_config c = cfg;
+        const int umax = c.tile_nnz > 0 ? c.tile_nnz : kCachedUmaxDefault;
+        complete_nb(c, nv, ldb, ldc, B, C, bf16);
+        if (c.lanes * c.vec < nv) return DSPMM_STATUS_NOT_SUPPORTED;     // one slab only
+        if (umax < 8 || umax > 4096 || umax % 8) return DSPMM_STATUS_NOT_SUPPORTED;
+        const int smax = cached_smax(umax);
+        if (nnz_cached_lds_bytes(umax, smax, nv, c.vec, bf16) > kMaxCachedLds)
+            return DSPMM_STATUS_NOT_SUPPORTED;
+        NbPlan *plan = nullptr;
+        dspmm_status_t s = get_plan(A, umax, &plan, kPlanCached);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        CachedLaunch a;
+        a.tiles = plan->d_tiles;
+        a.ntiles = plan->ntiles;
+        a.ucols = plan->d_ucols;
+        a.lidx = plan->d_lidx;
+        a.rowPtr = A->rowPtr;
+        a.colIdx = A->colIdx;
+        a.values = A->values;
+        a.B = B;
+        a.C = C;
+        a.ldb = ldb;
+        a.ldc = ldc;
+        a.nv = nv;
+        a.umax = umax;
+        a.smax = smax;
+        a.vw = c.vec;
+        a.lanes = c.lanes;
+        a.bf16 = bf16;
+        // workgroup size: cfg.unroll when given (64/128/256), else enough
+        // lane groups for the plan's mean rows per tile
+        if (cfg.unroll > 0) {
+            a.threads = cfg.unroll;
+        } else {
+            const double rows = plan->ntiles ? (double)A->nr / plan->ntiles : 1.0;
+            a.threads = 64;
+            while (a.threads < 256 && a.threads < rows * c.lanes) a.threads *= 2;
+        }
+        if (a.threads != 64 && a.threads != 128 && a.threads != 256) return DSPMM_STATUS_NOT_SUPPORTED;
+        if (a.lanes > a.threads) a.threads = a.lanes;
+        s = launch_nnz_cached(a, st);
+        if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
+        return s;
+    }
     case DSPMM_VARIANT_SPMV_TILED: {
         if (nv != 1) return DSPMM_STATUS_NOT_SUPPORTED;
         const int T = cfg.tile_nnz > 0 ? cfg.tile_nnz : kSpmvTile;
-        if (T < kSpmvRowMax || T > 16384) return DSPMM_STATUS_NOT_SUPPORTED;
+        if (!spmv_tile_ok(T)) return DSPMM_STATUS_NOT_SUPPORTED;
         NbPlan *plan = nullptr;
         dspmm_status_t s = get_plan(A, T, &plan, kPlanSpmv);
         if (s != DSPMM_STATUS_SUCCESS) return s;
@@ -402,10 +571,13 @@ dspmm_status_t check_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const voi
 }
 
 dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt,
-                     const dspmm_config *cfg) {
+                     const dspmm_config *cfg, bool learned) {
     if (cfg && cfg->variant != DSPMM_VARIANT_AUTO) return *cfg;
     auto it = A->tuned.find(std::make_tuple((int)layout, nv, (int)bt));
     if (it != A->tuned.end()) return it->second;
+    dspmm_config c;
+    if (learned && layout == DENSE_ROW_MAJOR && bt == DSPMM_R_32F && learned_select(A, nv, &c))
+        return c;
     return heuristic(A, layout, nv, bt);
 }
 
@@ -569,8 +741,15 @@ dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const
     if (A->nr == 0) return DSPMM_STATUS_SUCCESS;
     std::lock_guard<std::mutex> lk(A->mu);
     const dspmm_config c = resolve(A, layout, nv, btype, cfg);
-    return run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, workspace, workspace_bytes,
-                  as_hip(stream));
+    st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, workspace, workspace_bytes, as_hip(stream));
+    if (st == DSPMM_STATUS_NOT_SUPPORTED && (!cfg || cfg->variant == DSPMM_VARIANT_AUTO)) {
+        // a selected configuration these operands cannot run (alignment,
+        // LDS): the static default instead
+        const dspmm_config h = heuristic(A, layout, nv, btype);
+        st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, h, workspace, workspace_bytes,
+                    as_hip(stream));
+    }
+    return st;
 }
 
 dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
@@ -595,49 +774,81 @@ dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout, int nv, 
     if (st != DSPMM_STATUS_SUCCESS) return st;
     st = hip_status(hipStreamSynchronize(hs));
     if (st != DSPMM_STATUS_SUCCESS) return st;
-    std::vector<hipEvent_t> ev((size_t)reps + 1, nullptr), mid((size_t)reps, nullptr);
+    // Events only where an output asks for them: between launches for
+    // ms_per_launch, after the main kernel for ms_main_per_launch, and
+    // otherwise just around the whole timed region.
+    const bool per = ms_per_launch || ms_main_per_launch;
+    std::vector<hipEvent_t> ev(per ? (size_t)reps + 1 : 2, nullptr);
+    std::vector<hipEvent_t> mid(ms_main_per_launch ? (size_t)reps : 0, nullptr);
     for (auto &e : ev) if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventCreate(&e));
     for (auto &e : mid) if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventCreate(&e));
     if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventRecord(ev[0], hs));
     for (int i = 0; i < reps && st == DSPMM_STATUS_SUCCESS; i++) {
         {
             std::lock_guard<std::mutex> lk(A->mu);
-            st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, nullptr, 0, hs, mid[(size_t)i]);
+            st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, nullptr, 0, hs,
+                        mid.empty() ? nullptr : mid[(size_t)i]);
         }
-        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventRecord(ev[(size_t)i + 1], hs));
+        if (st == DSPMM_STATUS_SUCCESS && (per || i == reps - 1))
+            st = hip_status(hipEventRecord(ev[per ? (size_t)i + 1 : 1], hs));
     }
-    if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventSynchronize(ev[(size_t)reps]));
+    if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipEventSynchronize(ev.back()));
     if (st == DSPMM_STATUS_SUCCESS) {
-        double sum = 0;
-        for (int i = 0; i < reps && st == DSPMM_STATUS_SUCCESS; i++) {
+        float total = 0;
+        st = hip_status(hipEventElapsedTime(&total, ev.front(), ev.back()));
+        for (int i = 0; per && i < reps && st == DSPMM_STATUS_SUCCESS; i++) {
             float t = 0, tm = 0;
             st = hip_status(hipEventElapsedTime(&t, ev[(size_t)i], ev[(size_t)i + 1]));
-            if (st == DSPMM_STATUS_SUCCESS)
+            if (st == DSPMM_STATUS_SUCCESS && !mid.empty())
                 st = hip_status(hipEventElapsedTime(&tm, ev[(size_t)i], mid[(size_t)i]));
-            sum += t;
             if (ms_per_launch) ms_per_launch[i] = t;
             if (ms_main_per_launch) ms_main_per_launch[i] = tm;
         }
-        if (ms_mean) *ms_mean = (float)(sum / reps);
+        if (ms_mean) *ms_mean = total / (float)reps;
     }
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : mid) if (e) (void)hipEventDestroy(e);
     return st;
 }
 
-dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
-                              int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc, int reps,
-                              dspmm_config *best, float *best_ms, dspmm_stream_t stream) {
-    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
-    if (st != DSPMM_STATUS_SUCCESS) return st;
-    if (reps < 1) reps = 5;
-    // Candidate list: the native nnz-tiled kernel over tile sizes, unrolls
-    // and vector widths, then the four reference algorithm classes.
+}  // extern "C"
+
+namespace {
+
+// Candidate list of the autotuner: the native nnz-tiled kernel over vector
+// widths (each with the lane count that covers nv in one slab), tile sizes
+// around the default and both unrolls; the SpMV tiles when nv == 1; then
+// the four reference algorithm classes (FP32 B only).
+std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout layout, int nv,
+                                              int64_t ldb, int64_t ldc, const void *B,
+                                              const float *C, bool bf16) {
     std::vector<dspmm_config> cand;
-    const bool bf16 = btype == DSPMM_R_16BF;
+    if (layout == DENSE_ROW_MAJOR && nv > 1) {
+        // column-cached tiles, only where neighbouring rows share columns
+        double reuse;
+        {
+            std::lock_guard<std::mutex> lk(A->mu);
+            reuse = column_reuse(A);
+        }
+        if (reuse >= kCachedMinReuse) {
+            dspmm_config base{};
+            base.variant = DSPMM_VARIANT_NNZ_TILED;
+            complete_nb(base, nv, ldb, ldc, B, C, bf16);
+            for (int vw = base.vec; vw >= 1 && vw * 2 >= base.vec; vw /= 2) {
+                for (int umax : {64, 128, 256, 512}) {
+                    for (int nt : {64, 128, 256}) {
+                        dspmm_config c{};
+                        c.variant = DSPMM_VARIANT_NNZ_CACHED;
+                        c.vec = vw;
+                        c.tile_nnz = umax;
+                        c.unroll = nt;
+                        cand.push_back(c);
+                    }
+                }
+            }
+        }
+    }
     if (layout == DENSE_ROW_MAJOR) {
-        // Every feasible vector width, each with the lane count that covers
-        // nv in one slab; tile sizes around the default; both unrolls.
         dspmm_config base{};
         base.variant = DSPMM_VARIANT_NNZ_TILED;
         complete_nb(base, nv, ldb, ldc, B, C, bf16);
@@ -661,7 +872,7 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
         }
     }
     if (nv == 1) {
-        for (int T : {1024, 2048, 4096}) {
+        for (int T : {512, 1024, 2048, 4096}) {
             dspmm_config c{};
             c.variant = DSPMM_VARIANT_SPMV_TILED;
             c.tile_nnz = T;
@@ -676,24 +887,85 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
             cand.push_back(c);
         }
     }
-    float best_t = 1e30f;
-    dspmm_config best_c = heuristic(A, layout, nv, btype);
+    return cand;
+}
+
+// Median ms of each candidate over `reps` timed calls (after one warm-up);
+// a negative value marks a candidate these operands do not support.
+dspmm_status_t time_candidates(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                               int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc, int reps,
+                               const std::vector<dspmm_config> &cand, std::vector<float> &med,
+                               dspmm_stream_t stream) {
     std::vector<float> ms((size_t)reps);
-    for (const auto &c : cand) {
+    med.assign(cand.size(), -1.0f);
+    for (size_t i = 0; i < cand.size(); i++) {
         float mean = 0;
-        st = dspmm_benchmark_ex(A, layout, nv, B, ldb, btype, C, ldc, &c, 1, reps, &mean, ms.data(),
-                                nullptr, stream);
+        const dspmm_status_t st = dspmm_benchmark_ex(A, layout, nv, B, ldb, btype, C, ldc,
+                                                     &cand[i], 1, reps, &mean, ms.data(), nullptr,
+                                                     stream);
         if (st == DSPMM_STATUS_NOT_SUPPORTED) continue;
         if (st != DSPMM_STATUS_SUCCESS) return st;
-        const float med = median_of(ms);
-        if (med < best_t) { best_t = med; best_c = c; }
+        med[i] = median_of(ms);
     }
+    return DSPMM_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                              int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc, int reps,
+                              dspmm_config *best, float *best_ms, dspmm_stream_t stream) {
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (reps < 1) reps = 5;
+    const std::vector<dspmm_config> cand =
+        autotune_candidates(A, layout, nv, ldb, ldc, B, C, btype == DSPMM_R_16BF);
+    std::vector<float> med;
+    st = time_candidates(A, layout, nv, B, ldb, btype, C, ldc, reps, cand, med, stream);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    float best_t = 1e30f;
+    dspmm_config best_c;
+    {
+        std::lock_guard<std::mutex> lk(A->mu);
+        best_c = heuristic(A, layout, nv, btype);
+    }
+    for (size_t i = 0; i < cand.size(); i++)
+        if (med[i] >= 0 && med[i] < best_t) {
+            best_t = med[i];
+            best_c = cand[i];
+        }
     {
         std::lock_guard<std::mutex> lk(A->mu);
         A->tuned[std::make_tuple((int)layout, nv, (int)btype)] = best_c;
     }
     if (best) *best = best_c;
     if (best_ms) *best_ms = best_t;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_autotune_table(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                                    int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,
+                                    int reps, dspmm_config *configs, float *ms, int capacity,
+                                    int *count, dspmm_stream_t stream) {
+    if (!count || capacity < 0 || (capacity > 0 && (!configs || !ms)))
+        return DSPMM_STATUS_INVALID_VALUE;
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (reps < 1) reps = 5;
+    const std::vector<dspmm_config> cand =
+        autotune_candidates(A, layout, nv, ldb, ldc, B, C, btype == DSPMM_R_16BF);
+    *count = (int)cand.size();
+    if (capacity == 0) return DSPMM_STATUS_SUCCESS;
+    if (capacity < (int)cand.size()) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
+    std::vector<float> med;
+    st = time_candidates(A, layout, nv, B, ldb, btype, C, ldc, reps, cand, med, stream);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    for (size_t i = 0; i < cand.size(); i++) {
+        configs[i] = cand[i];
+        ms[i] = med[i];
+    }
     return DSPMM_STATUS_SUCCESS;
 }
 

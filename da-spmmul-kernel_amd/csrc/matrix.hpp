@@ -20,6 +20,10 @@ struct NbPlan {
     int4 *d_splits = nullptr;         // {0, row, first_slot, nchunks}
     std::vector<int4> h_tiles, h_splits;
     size_t bytes = 0;
+    // column-cached plans only: per-tile distinct columns, per-nonzero position
+    int *d_ucols = nullptr;
+    uint16_t *d_lidx = nullptr;
+    double reuse = 0;                 // nonzeros per B row read from L2
 };
 
 // LDS budget of one nnz-tiled workgroup (keeps two workgroups per CU).
@@ -32,7 +36,16 @@ void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &t
 void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
                       std::vector<int4> &tiles, std::vector<int4> &splits, int &nslots);
 
-enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1 };
+// Column-cached plan: tiles of whole rows with at most umax distinct columns,
+// at most smax nonzeros and kCachedRowsMax rows; a row that does not fit alone
+// becomes a direct tile {row, -1, p_begin, u}.  tiles has a sentinel entry
+// whose .w is the total number of listed columns.
+void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, int nr, int nc,
+                        int umax, int smax, std::vector<int4> &tiles, std::vector<int> &ucols,
+                        std::vector<uint16_t> &lidx, double &reuse);
+
+enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1, kPlanCached = 2 };
+constexpr size_t kMaxCachedLds = 80 * 1024;   // two column-cached workgroups per CU
 
 }  // namespace dspmm
 
@@ -45,7 +58,8 @@ struct dspmm_matrix_s {
     double mean = 0, cv = 0;
     size_t base_bytes = 0;
     std::mutex mu;
-    std::map<int, dspmm::NbPlan> plans;                            // by 2 * tile size + kind
+    std::map<int, dspmm::NbPlan> plans;                            // by 4 * tile size + kind
+    double reuse16 = -1;     // sampled nonzeros per distinct column over 16-row windows (-1: not yet)
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     float *carry = nullptr;
     size_t carry_bytes = 0;
@@ -62,10 +76,20 @@ namespace dspmm {
 
 // All of these expect the caller to hold A->mu.
 dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind = kPlanNnzTiled);
+// Column reuse of neighbouring rows, sampled (computed once per matrix).
+double column_reuse(dspmm_matrix_s *A);
+int cached_smax(int umax);
 int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,
                  bool bf16);
+// The configuration a call runs: cfg unless AUTO, else the tuned one cached
+// for (layout, nv, bt), else (with `learned`, row-major FP32 only) the
+// nearest-neighbour selector's pick, else the static heuristic.
 dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt,
-                     const dspmm_config *cfg);
+                     const dspmm_config *cfg, bool learned = true);
+// The static default (no timing, no corpus).
+dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt);
+// Nearest-neighbour selection over the measured corpus (selector.cpp).
+bool learned_select(dspmm_matrix_s *A, int nv, dspmm_config *out);
 
 }  // namespace dspmm

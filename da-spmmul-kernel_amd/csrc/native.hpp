@@ -30,6 +30,33 @@ struct ItemDesc {
     int nsplits;
 };
 
+// One launch of the column-cached kernel (k_nnz_cached).  Tiles are
+// {row_begin, row_end, p_begin, u_begin}; tiles[ntiles].w is the total
+// number of cached columns.  A tile with row_end < 0 holds the single row
+// row_begin, whose columns exceed the cache: it gathers B directly.
+struct CachedLaunch {
+    const int4 *tiles = nullptr;
+    int ntiles = 0;
+    const int *ucols = nullptr;        // the distinct columns of each tile, tile after tile
+    const uint16_t *lidx = nullptr;    // per nonzero: position of its column in its tile's list
+    const int *rowPtr = nullptr;
+    const int *colIdx = nullptr;
+    const float *values = nullptr;
+    const void *B = nullptr;
+    float *C = nullptr;
+    int64_t ldb = 0, ldc = 0;
+    int nv = 0;
+    int umax = 0;                      // cached B rows per tile (LDS rows)
+    int smax = 0;                      // nonzeros per tile
+    int vw = 0, lanes = 0;
+    int threads = 256;                 // workgroup size: 64, 128 or 256
+    bool bf16 = false;
+};
+dspmm_status_t launch_nnz_cached(const CachedLaunch &a, hipStream_t st);
+// LDS bytes of one k_nnz_cached workgroup.
+size_t nnz_cached_lds_bytes(int umax, int smax, int nv, int vw, bool bf16);
+constexpr int kCachedRowsMax = 256;    // rows per cached tile
+
 // Arguments of one single-matrix nnz-tiled launch (plus its split combine).
 struct NbLaunch {
     ItemDesc item{};
@@ -63,7 +90,10 @@ dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st);
 // SpMV (nv == 1) tiles: item.S is the tile size T, item.tiles a plan made
 // by build_spmv_tiles.
 dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st);
-constexpr int kSpmvRowMax = 256;   // longest row summed by one thread
+constexpr int kSpmvRowMax = 256;   // longest row of a multi-row SpMV tile
+constexpr int kSpmvThreadRow = 16; // longest row summed by one thread
+constexpr int kSpmvLongTiles = 16; // rows up to 16 T nonzeros get one workgroup
+bool spmv_tile_ok(int T);          // SpMV tile sizes with a kernel: 512 .. 4096
 size_t nnz_tiled_lds_bytes(int lanes, int S);
 int item_class(int vw, int lanes);
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);

@@ -325,44 +325,101 @@ __global__ __launch_bounds__(kBlock) void k_split_combine_batched(const ItemDesc
 }
 
 // ---- SpMV (nv == 1) -----------------------------------------------------------
-// One workgroup per tile of at most T nonzeros.  Phase 1 computes every
-// product a_p * x[col_p] into LDS with coalesced loads of (col, val) and
-// independent gathers of x.  Phase 2: in a tile of whole rows no longer
-// than kSpmvRowMax, thread i sums row r0+i from LDS in nonzero order --
-// bit-exact with the host loop; a tile holding (part of) one longer row is
-// reduced by the workgroup in a fixed tree order and written to y (one
-// chunk) or to its carry slot (several chunks, combined afterwards).
-template <typename TB>
+// One workgroup per tile.  A tile of whole rows (each no longer than
+// kSpmvRowMax, at most T = E * 256 nonzeros): every thread first issues its
+// E (col, val) loads and its row bounds, then E independent gathers of x,
+// and writes the products to LDS.  After one barrier, thread i sums row
+// r0+i from LDS in nonzero order if the row has at most kSpmvThreadRow
+// nonzeros -- bit-exact with the host loop; longer rows of the tile are
+// queued in LDS and taken one per wavefront: lane l sums products l, l+64,
+// ... in order, then a fixed butterfly.  A tile of one longer row (up to
+// kSpmvLongTiles * T nonzeros) or of one T-chunk of a still longer row is
+// reduced by the workgroup: thread t sums products t, t+256, t+512, ... in
+// that order, then a pairwise tree; the result goes to y (whole row) or to
+// the chunk's carry slot.
+template <typename TB, int E>
 __global__ __launch_bounds__(kBlock) void k_spmv_tiled(ItemDesc d, int ntiles) {
-    extern __shared__ float prod[];
+    __shared__ float prod[E * kBlock];
     __shared__ float red[kBlock];
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int4 tile = d.tiles[b];
-    const int pb = tile.z, pe = tile.w, n = pe - pb;
+    const int4 tile = d.tiles[blockIdx.x];
+    const int pb = tile.z, n = tile.w - tile.z;
     const int t = threadIdx.x;
     const TB *__restrict__ x = reinterpret_cast<const TB *>(d.B);
-    for (int i = t; i < n; i += kBlock) {
-        const int c = ld_stream(d.colIdx + pb + i);
-        const float a = ld_stream(d.values + pb + i);
-        float xv[1];
-        load_b<TB, 1>(xv, x + (int64_t)c * d.ldb);
-        prod[i] = a * xv[0];
-    }
-    __syncthreads();
+    const int *__restrict__ ci = d.colIdx + pb;
+    const float *__restrict__ va = d.values + pb;
     const bool tree = tile.y < 0 || (tile.y - tile.x == 1 && n > kSpmvRowMax);
     if (!tree) {
+        __shared__ int nlong;
+        __shared__ int4 lrange[kBlock];
+        if (t == 0) nlong = 0;
         const int r = tile.x + t;
+        int s0 = 0, s1 = 0;
         if (r < tile.y) {
-            const int s0 = d.rowPtr[r] - pb, s1 = d.rowPtr[r + 1] - pb;
+            s0 = d.rowPtr[r] - pb;
+            s1 = d.rowPtr[r + 1] - pb;
+        }
+        int c[E];
+        float a[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int i = t + k * kBlock;
+            c[k] = 0;
+            a[k] = 0.f;
+            if (i < n) {
+                c[k] = ld_stream(ci + i);
+                a[k] = ld_stream(va + i);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int i = t + k * kBlock;
+            if (i < n) {
+                float xv[1];
+                load_b<TB, 1>(xv, x + (int64_t)c[k] * d.ldb);
+                prod[i] = a[k] * xv[0];
+            }
+        }
+        __syncthreads();
+        if (r < tile.y) {
+            if (s1 - s0 <= kSpmvThreadRow) {
+                float s = 0.f;
+                for (int p = s0; p < s1; p++) s = s + prod[p];
+                d.C[(int64_t)r * d.ldc] = s;
+            } else {
+                lrange[atomicAdd(&nlong, 1)] = make_int4(s0, s1, r, 0);
+            }
+        }
+        __syncthreads();
+        const int wv = t / kWave, ln = t % kWave;
+        for (int k = wv; k < nlong; k += kBlock / kWave) {      // wave-uniform
+            const int4 q = lrange[k];
             float s = 0.f;
-            for (int p = s0; p < s1; p++) s = s + prod[p];
-            d.C[(int64_t)r * d.ldc] = s;
+            for (int p = q.x + ln; p < q.y; p += kWave) s = s + prod[p];
+            s = butterfly_sum<1, kWave>(s);
+            if (ln == 0) d.C[(int64_t)q.z * d.ldc] = s;
         }
         return;
     }
-    // fixed-order tree: thread t sums t, t+256, ... then pairwise halving
     float s = 0.f;
-    for (int i = t; i < n; i += kBlock) s = s + prod[i];
+    int i = t;
+    for (; i + 3 * kBlock < n; i += 4 * kBlock) {
+        int c[4];
+        float a[4], xv[4][1];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c[k] = ld_stream(ci + i + k * kBlock);
+            a[k] = ld_stream(va + i + k * kBlock);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) load_b<TB, 1>(xv[k], x + (int64_t)c[k] * d.ldb);
+#pragma unroll
+        for (int k = 0; k < 4; k++) s = s + a[k] * xv[k][0];
+    }
+    for (; i < n; i += kBlock) {
+        float xv[1];
+        load_b<TB, 1>(xv, x + (int64_t)ld_stream(ci + i) * d.ldb);
+        s = s + ld_stream(va + i) * xv[0];
+    }
     red[t] = s;
     __syncthreads();
     for (int h = kBlock / 2; h > 0; h >>= 1) {
@@ -372,6 +429,137 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiled(ItemDesc d, int ntiles) {
     if (t == 0) {
         if (tile.y < 0) d.carry[(int64_t)(-1 - tile.y) * d.ldcarry] = red[0];
         else d.C[(int64_t)tile.x * d.ldc] = red[0];
+    }
+}
+
+// ---- column-cached SpMM (row-major) -------------------------------------------
+// For matrices whose neighbouring rows share columns (stencils, banded
+// matrices): a host plan cuts the rows into tiles of whole rows whose
+// distinct columns number at most umax, and gives every nonzero the position
+// of its column in its tile's column list.  One workgroup per tile copies the
+// tile's B rows into LDS once (coalesced, 16 B per lane), stages the tile's
+// (position, value) pairs and row ends, and then every row is summed by a
+// group of L lanes x VW columns from LDS, in nonzero order with a separately
+// rounded multiply and add -- bit-exact with the host loop.  Each B row is
+// read from L2 once per tile instead of once per nonzero.  A row whose own
+// columns exceed umax is a "direct" tile: the G groups take contiguous
+// chunks of it, gather B from global memory, and group 0 adds the G
+// partial rows in group order.
+template <int VW, typename TB>
+__device__ __forceinline__ void copy_raw(TB *dst, const TB *src) {
+    constexpr int bytes = VW * (int)sizeof(TB);
+    if constexpr (bytes == 16) *reinterpret_cast<uint4 *>(dst) = *reinterpret_cast<const uint4 *>(src);
+    else if constexpr (bytes == 8) *reinterpret_cast<uint2 *>(dst) = *reinterpret_cast<const uint2 *>(src);
+    else if constexpr (bytes == 4) *reinterpret_cast<uint32_t *>(dst) = *reinterpret_cast<const uint32_t *>(src);
+    else *reinterpret_cast<uint16_t *>(dst) = *reinterpret_cast<const uint16_t *>(src);
+}
+
+template <typename TB, int VW>
+__device__ __forceinline__ void widen(float (&r)[VW], const TB *p) {
+    if constexpr (sizeof(TB) == 4) {
+#pragma unroll
+        for (int k = 0; k < VW; k++) r[k] = reinterpret_cast<const float *>(p)[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW; k++)
+            r[k] = __uint_as_float((uint32_t)reinterpret_cast<const uint16_t *>(p)[k] << 16);
+    }
+}
+
+// LDS row stride of the cached B rows: nv rounded up to VW, plus VW padding
+// elements so that groups reading different rows start on different banks.
+__host__ __device__ inline int cached_stride(int nv, int vw) { return (nv + vw - 1) / vw * vw + vw; }
+
+template <typename TB, int VW, int L, int NT>
+__global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
+    constexpr int G = NT / L;
+    extern __shared__ __align__(16) unsigned char csm[];
+    const int t = threadIdx.x;
+    const int g = t / L;
+    const int col0 = (t % L) * VW;
+    const int ls = cached_stride(a.nv, VW);
+    TB *Bl = reinterpret_cast<TB *>(csm);
+    float *lv = reinterpret_cast<float *>(csm + (size_t)a.umax * ls * sizeof(TB));
+    uint16_t *li = reinterpret_cast<uint16_t *>(lv + a.smax);
+    int *rend = reinterpret_cast<int *>(li + ((a.smax + 1) & ~1));
+    const int ti = xcd_remap(blockIdx.x, gridDim.x);
+    const int4 tile = a.tiles[ti];
+    const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B);
+
+    if (tile.y < 0) {                                  // direct tile: one long row
+        const int row = tile.x, p0 = tile.z, p1 = a.rowPtr[row + 1];
+        const int len = p1 - p0, ch = (len + G - 1) / G;
+        const int c0 = p0 + min(len, g * ch), c1 = p0 + min(len, (g + 1) * ch);
+        float acc[VW];
+#pragma unroll
+        for (int k = 0; k < VW; k++) acc[k] = 0.f;
+        if (col0 < a.nv) {
+            for (int p = c0; p < c1; p++) {
+                float bv[VW];
+                load_b<TB, VW>(bv, B + (int64_t)ld_stream(a.colIdx + p) * a.ldb + col0);
+                const float v = ld_stream(a.values + p);
+#pragma unroll
+                for (int k = 0; k < VW; k++) acc[k] = acc[k] + v * bv[k];
+            }
+        }
+        float *part = reinterpret_cast<float *>(csm);   // G partial rows of stride ls
+        if (col0 < a.nv) {
+#pragma unroll
+            for (int k = 0; k < VW; k++) part[g * ls + col0 + k] = acc[k];
+        }
+        __syncthreads();
+        if (g == 0 && col0 < a.nv) {
+            float s[VW];
+#pragma unroll
+            for (int k = 0; k < VW; k++) s[k] = part[col0 + k];
+            for (int q = 1; q < G; q++) {
+#pragma unroll
+                for (int k = 0; k < VW; k++) s[k] = s[k] + part[q * ls + col0 + k];
+            }
+            store_vec_nt<VW>(a.C + (int64_t)row * a.ldc + col0, s);
+        }
+        return;
+    }
+
+    const int r0 = tile.x, r1 = tile.y, p0 = tile.z;
+    const int u0 = tile.w, nu = a.tiles[ti + 1].w - u0;
+    const int np = a.rowPtr[r1] - p0;
+    // stage the tile's B rows (group g copies rows g, g+G, ...)
+    for (int u = g; u < nu; u += G) {
+        if (col0 < a.nv) copy_raw<VW>(Bl + u * ls + col0, B + (int64_t)a.ucols[u0 + u] * a.ldb + col0);
+    }
+    for (int i = t; i < np; i += NT) {
+        lv[i] = ld_stream(a.values + p0 + i);
+        li[i] = a.lidx[p0 + i];
+    }
+    for (int i = t; i < r1 - r0; i += NT) rend[i] = a.rowPtr[r0 + 1 + i] - p0;
+    __syncthreads();
+    if (col0 >= a.nv) return;
+    for (int rr = g; rr < r1 - r0; rr += G) {
+        const int s0 = rr == 0 ? 0 : rend[rr - 1], s1 = rend[rr];
+        float acc[VW];
+#pragma unroll
+        for (int k = 0; k < VW; k++) acc[k] = 0.f;
+        int p = s0;
+        for (; p + 3 < s1; p += 4) {
+            float bv[4][VW];
+#pragma unroll
+            for (int u = 0; u < 4; u++) widen<TB, VW>(bv[u], Bl + (int)li[p + u] * ls + col0);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float v = lv[p + u];
+#pragma unroll
+                for (int k = 0; k < VW; k++) acc[k] = acc[k] + v * bv[u][k];
+            }
+        }
+        for (; p < s1; p++) {
+            float bv[VW];
+            widen<TB, VW>(bv, Bl + (int)li[p] * ls + col0);
+            const float v = lv[p];
+#pragma unroll
+            for (int k = 0; k < VW; k++) acc[k] = acc[k] + v * bv[k];
+        }
+        store_vec_nt<VW>(a.C + (int64_t)(r0 + rr) * a.ldc + col0, acc);
     }
 }
 
@@ -515,14 +703,91 @@ dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st) {
     return DSPMM_STATUS_SUCCESS;
 }
 
+namespace {
+template <typename TB>
+dspmm_status_t launch_spmv_e(const NbLaunch &a, hipStream_t st) {
+    const dim3 g(a.ntiles), b(kBlock);
+    switch (a.item.S) {
+    case 512: hipLaunchKernelGGL((k_spmv_tiled<TB, 2>), g, b, 0, st, a.item, a.ntiles); break;
+    case 1024: hipLaunchKernelGGL((k_spmv_tiled<TB, 4>), g, b, 0, st, a.item, a.ntiles); break;
+    case 2048: hipLaunchKernelGGL((k_spmv_tiled<TB, 8>), g, b, 0, st, a.item, a.ntiles); break;
+    case 4096: hipLaunchKernelGGL((k_spmv_tiled<TB, 16>), g, b, 0, st, a.item, a.ntiles); break;
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+    return launch_status();
+}
+}  // namespace
+
+size_t nnz_cached_lds_bytes(int umax, int smax, int nv, int vw, bool bf16) {
+    const size_t ls = (size_t)cached_stride(nv, vw);
+    const size_t eb = bf16 ? 2 : 4;
+    const size_t brows = (size_t)umax * ls * eb;
+    // direct tiles: G = NT / L partial rows of ls floats, ls <= (L + 1) * vw, NT <= 256
+    const size_t direct = (size_t)2 * kBlock * (size_t)vw * 4;
+    const size_t stage = (size_t)smax * 4 + (size_t)((smax + 1) & ~1) * 2 + (size_t)kCachedRowsMax * 4;
+    return std::max(brows, direct) + stage;
+}
+
+namespace {
+template <typename TB, int VW, int L, int NT>
+dspmm_status_t launch_cached_vln(const CachedLaunch &a, hipStream_t st) {
+    if constexpr (L > NT) {
+        return DSPMM_STATUS_NOT_SUPPORTED;
+    } else {
+        const size_t lds = nnz_cached_lds_bytes(a.umax, a.smax, a.nv, VW, sizeof(TB) == 2);
+        hipLaunchKernelGGL((k_nnz_cached<TB, VW, L, NT>), dim3(a.ntiles), dim3(NT), lds, st, a);
+        return launch_status();
+    }
+}
+template <typename TB, int VW, int L>
+dspmm_status_t launch_cached_vl(const CachedLaunch &a, hipStream_t st) {
+    switch (a.threads) {
+    case 64: return launch_cached_vln<TB, VW, L, 64>(a, st);
+    case 128: return launch_cached_vln<TB, VW, L, 128>(a, st);
+    case 256: return launch_cached_vln<TB, VW, L, 256>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+template <typename TB, int VW>
+dspmm_status_t launch_cached_l(const CachedLaunch &a, hipStream_t st) {
+    switch (a.lanes) {
+    case 1: return launch_cached_vl<TB, VW, 1>(a, st);
+    case 2: return launch_cached_vl<TB, VW, 2>(a, st);
+    case 4: return launch_cached_vl<TB, VW, 4>(a, st);
+    case 8: return launch_cached_vl<TB, VW, 8>(a, st);
+    case 16: return launch_cached_vl<TB, VW, 16>(a, st);
+    case 32: return launch_cached_vl<TB, VW, 32>(a, st);
+    case 64: return launch_cached_vl<TB, VW, 64>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+}  // namespace
+
+dspmm_status_t launch_nnz_cached(const CachedLaunch &a, hipStream_t st) {
+    if (a.ntiles == 0) return DSPMM_STATUS_SUCCESS;
+    if (a.lanes * a.vw < a.nv) return DSPMM_STATUS_NOT_SUPPORTED;
+    if (a.bf16) {
+        switch (a.vw) {
+        case 8: return launch_cached_l<uint16_t, 8>(a, st);
+        case 4: return launch_cached_l<uint16_t, 4>(a, st);
+        case 2: return launch_cached_l<uint16_t, 2>(a, st);
+        case 1: return launch_cached_l<uint16_t, 1>(a, st);
+        default: return DSPMM_STATUS_NOT_SUPPORTED;
+        }
+    }
+    switch (a.vw) {
+    case 4: return launch_cached_l<float, 4>(a, st);
+    case 2: return launch_cached_l<float, 2>(a, st);
+    case 1: return launch_cached_l<float, 1>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+
+bool spmv_tile_ok(int T) { return T == 512 || T == 1024 || T == 2048 || T == 4096; }
+
 dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
     if (a.ntiles == 0) return DSPMM_STATUS_SUCCESS;
-    const size_t lds = sizeof(float) * (size_t)a.item.S;
-    if (a.bf16)
-        hipLaunchKernelGGL(k_spmv_tiled<uint16_t>, dim3(a.ntiles), dim3(kBlock), lds, st, a.item, a.ntiles);
-    else
-        hipLaunchKernelGGL(k_spmv_tiled<float>, dim3(a.ntiles), dim3(kBlock), lds, st, a.item, a.ntiles);
-    dspmm_status_t s = launch_status();
+    dspmm_status_t s = a.bf16 ? launch_spmv_e<uint16_t>(a, st) : launch_spmv_e<float>(a, st);
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {

@@ -47,8 +47,9 @@ DSPMM_R_32F, DSPMM_R_16BF = 0, 1
 VARIANT_AUTO, VARIANT_NNZ_TILED = 0, 1
 VARIANT_CSR_SCALAR, VARIANT_CSR_VECTOR, VARIANT_COO_SCALAR, VARIANT_COO_VECTOR = 2, 3, 4, 5
 VARIANT_SPMV_TILED = 6
+VARIANT_NNZ_CACHED = 7
 VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
-                 4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled"}
+                 4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled", 7: "nnz_cached"}
 MATRIX_WITH_COO = 0x1
 
 STATUS_SUCCESS = 0
@@ -188,6 +189,8 @@ def _declare(lib):
         "dspmm_spmm_ex_resolve": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(Config)], _ST),
         "dspmm_autotune": ([_P, _I, _I, _P, _I64, _I, _P, _I64, _I, C.POINTER(Config),
                             C.POINTER(C.c_float), _P], _ST),
+        "dspmm_autotune_table": ([_P, _I, _I, _P, _I64, _I, _P, _I64, _I, C.POINTER(Config),
+                                  C.POINTER(C.c_float), _I, C.POINTER(C.c_int), _P], _ST),
         "dspmm_benchmark_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _I, _I,
                                 C.POINTER(C.c_float), C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _P], _ST),
@@ -457,19 +460,36 @@ class Matrix:
                                    reps, C.byref(best), C.byref(ms), stream), "autotune")
         return best, ms.value
 
+    def autotune_table(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, reps=5,
+                       ldb=None, ldc=None, stream=None):
+        """Every autotuner candidate with its median ms (None: not supported)."""
+        ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
+        ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
+        n = C.c_int()
+        args = (self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out), ldc, reps)
+        check(lib().dspmm_autotune_table(*args, None, None, 0, C.byref(n), stream),
+              "autotune_table")
+        cfgs = (Config * n.value)()
+        ms = (C.c_float * n.value)()
+        check(lib().dspmm_autotune_table(*args, cfgs, ms, n.value, C.byref(n), stream),
+              "autotune_table")
+        return [(cfgs[i], ms[i] if ms[i] >= 0 else None) for i in range(n.value)]
+
     def benchmark(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, config=None,
-                  warmup=5, reps=50, ldb=None, ldc=None, stream=None):
-        """Returns (mean ms per call, per-call ms list, per-call main-kernel ms list)."""
+                  warmup=5, reps=50, ldb=None, ldc=None, stream=None, per_call=True):
+        """Returns (mean ms per call, per-call ms list, per-call main-kernel ms
+        list).  per_call=False times the region only (no events between
+        calls); the two lists are then empty."""
         ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
         ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
         mean = C.c_float()
-        arr = (C.c_float * reps)()
-        main = (C.c_float * reps)()
+        arr = (C.c_float * reps)() if per_call else None
+        main = (C.c_float * reps)() if per_call else None
         check(lib().dspmm_benchmark_ex(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out),
                                        ldc, C.byref(config) if config is not None else None,
                                        warmup, reps, C.byref(mean), arr, main, stream),
               "benchmark_ex")
-        return mean.value, list(arr), list(main)
+        return mean.value, (list(arr) if per_call else []), (list(main) if per_call else [])
 
     def close(self):
         if getattr(self, "handle", None) and _lib is not None:
@@ -492,7 +512,7 @@ def to_bf16_device(src: DeviceArray, stream=None) -> DeviceArray:
     return dst
 
 
-BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PLAIN = 0, 1, 2
+BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PLAIN, BATCH_PLAIN_OWN = 0, 1, 2, 3
 
 
 class Batch:

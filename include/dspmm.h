@@ -243,9 +243,17 @@ typedef enum dspmm_variant {
     DSPMM_VARIANT_CSR_VECTOR = 3,
     DSPMM_VARIANT_COO_SCALAR = 4,
     DSPMM_VARIANT_COO_VECTOR = 5,
-    DSPMM_VARIANT_SPMV_TILED = 6 /* nv == 1 only: workgroup per nnz tile, products
+    DSPMM_VARIANT_SPMV_TILED = 6,/* nv == 1 only: workgroup per nnz tile, products
                                     in LDS, per-row sums (row-exact), tree for
                                     long rows; tile_nnz = tile size */
+    DSPMM_VARIANT_NNZ_CACHED = 7 /* row-major, lanes * vec >= nv: tiles of whole
+                                    rows whose distinct columns' B rows are
+                                    staged in LDS once per tile (matrices whose
+                                    neighbouring rows share columns); rows are
+                                    summed like the host loop (row-exact);
+                                    tile_nnz = distinct columns per tile,
+                                    unroll = workgroup size (64/128/256,
+                                    0 = from the plan's rows per tile) */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {
@@ -283,12 +291,25 @@ DSPMM_API dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, in
                                         float *C, int64_t ldc, int reps,
                                         dspmm_config *best, float *best_ms,
                                         dspmm_stream_t stream);
+/* The autotuner's whole table: every candidate it would time for these
+ * operands and its median ms over `reps` calls (negative: not supported).
+ * With capacity == 0 only *count is set; a capacity below *count gives
+ * DSPMM_STATUS_WORKSPACE_TOO_SMALL.  Nothing is cached.  This is the data a
+ * sweep records and the selector is fitted on. */
+DSPMM_API dspmm_status_t dspmm_autotune_table(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                              const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                              float *C, int64_t ldc, int reps,
+                                              dspmm_config *configs, float *ms, int capacity,
+                                              int *count, dspmm_stream_t stream);
 
-/* Timed loop over dspmm_spmm_ex (warmup untimed, then reps launches with an
- * event between launches).  ms_per_launch[reps] gets each call's time;
- * ms_main_per_launch[reps] (may be NULL) the time of its main kernel only,
- * i.e. without the split-row combine that follows it (an event is recorded
- * between the two; no work is skipped). */
+/* Timed loop over dspmm_spmm_ex: warmup untimed calls, then reps timed
+ * ones.  *ms_mean is the timed region's length / reps.  ms_per_launch[reps]
+ * (may be NULL) gets each call's time, ms_main_per_launch[reps] (may be
+ * NULL) the time of its main kernel only, without the split-row combine that
+ * follows it.  Events are recorded only where an output asks for them:
+ * between calls when either array is given, and between a call's main
+ * kernel and its combine for ms_main_per_launch; with both NULL only the
+ * region is bracketed.  No work is skipped in any mode. */
 DSPMM_API dspmm_status_t dspmm_benchmark_ex(dspmm_matrix_t A, DenseLayout layout, int nv,
                                             const void *B, int64_t ldb, dspmm_dtype_t btype,
                                             float *C, int64_t ldc, const dspmm_config *cfg,
@@ -350,8 +371,11 @@ DSPMM_API dspmm_status_t dspmm_batch_run_plain(dspmm_batch_t b, dspmm_stream_t s
  * and any captured graph stay valid.  Ordered on `stream`. */
 DSPMM_API dspmm_status_t dspmm_batch_set_operands(dspmm_batch_t b, int index, const void *B,
                                                   float *C, dspmm_stream_t stream);
-/* mode 0: dspmm_batch_run, 1: dspmm_batch_graph_launch, 2: dspmm_batch_run_plain.
- * Same timing contract as dspmm_benchmark_ex. */
+/* mode 0: dspmm_batch_run, 1: dspmm_batch_graph_launch, 2: dspmm_batch_run_plain
+ * (one launch per item with the batch's configuration), 3: one launch per item
+ * with the item's own AUTO configuration (tuned, else selected, else the
+ * static default) -- the plain baseline at each matrix's best known kernel.
+ * ms_per_launch[reps] (may be NULL) gets each pass's time. */
 DSPMM_API dspmm_status_t dspmm_benchmark_batch(dspmm_batch_t b, int mode, int warmup, int reps,
                                                float *ms_mean, float *ms_per_launch,
                                                dspmm_stream_t stream);

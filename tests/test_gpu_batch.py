@@ -62,10 +62,10 @@ def test_batched_equals_single_and_oracle():
     assert info["units"] > 0 and info["blocks"] % 8 == 0
     b.run()
     for it in items:
-        # single-matrix path with the config the batch resolved: the matrix's
-        # own nnz-tiled config (nv == 1 would otherwise pick the SpMV kernel)
-        cfg = D.make_config(D.VARIANT_NNZ_TILED) if it.nv == 1 else None
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=cfg)
+        # single-matrix path with the config the batch resolved: an untuned
+        # matrix's static nnz-tiled default (AUTO on a single launch may pick
+        # another kernel or tile)
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=D.make_config(D.VARIANT_NNZ_TILED))
     D.synchronize()
     for i, it in enumerate(items):
         out, ref = it.d_C.numpy().reshape(-1), it.d_C_ref.numpy().reshape(-1)
@@ -102,7 +102,8 @@ def test_graph_replays_follow_rebound_operands():
         b.graph_launch()
         D.synchronize()
         # eager work between replays
-        items[0].m.spmm(news[0][1], items[0].d_C_ref, items[0].nv)
+        items[0].m.spmm(news[0][1], items[0].d_C_ref, items[0].nv,
+                        config=D.make_config(D.VARIANT_NNZ_TILED))
         D.synchronize()
         assert np.array_equal(bits(items[0].d_C_ref.numpy()), bits(items[0].d_C.numpy()))
         for i, it in enumerate(items):
@@ -116,7 +117,8 @@ def test_batched_bf16():
     b.run()
     D.synchronize()
     for i, it in enumerate(items):
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv, btype=D.DSPMM_R_16BF)
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, btype=D.DSPMM_R_16BF,
+                  config=D.make_config(D.VARIANT_NNZ_TILED))
     D.synchronize()
     for i, it in enumerate(items):
         out = it.d_C.numpy().reshape(-1)
@@ -128,7 +130,7 @@ def test_batched_bf16():
 def test_benchmark_modes_run():
     items = make_items(16, seed=44)
     b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
-    for mode in (D.BATCH_PERSISTENT, D.BATCH_GRAPH, D.BATCH_PLAIN):
+    for mode in (D.BATCH_PERSISTENT, D.BATCH_GRAPH, D.BATCH_PLAIN, D.BATCH_PLAIN_OWN):
         ms, per = b.benchmark(mode, warmup=1, reps=3)
         note(f"mode {mode}: {ms * 1e3:.1f} us")
         assert ms > 0 and len(per) == 3

@@ -192,9 +192,9 @@ def test_matrix_from_device_arrays_and_describe():
 
 
 @pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20", "c2"])
-@pytest.mark.parametrize("T", [1024, 2048, 4096])
+@pytest.mark.parametrize("T", [512, 1024, 2048, 4096])
 def test_spmv_tiled(name, T):
-    """SpMV kernel: rows of <= 256 nonzeros are summed like the host loop (bit
+    """SpMV kernel: rows of <= 16 nonzeros are summed like the host loop (bit
     for bit); longer rows are tree-reduced (and combined across tiles)."""
     c, m = matrix(name)
     h32, c64, ab = c.ref(RM, 1)
@@ -203,8 +203,8 @@ def test_spmv_tiled(name, T):
         what = f"{name} spmv T={T} layout={layout}"
         note(what)
         out = run_native(c, m, 1, layout=layout, config=cfg)
-        short = c.rowlen <= 256
-        assert_bitwise(out[short], h32[short], what + " (rows <= 256)")
+        short = c.rowlen <= 16
+        assert_bitwise(out[short], h32[short], what + " (rows <= 16)")
         assert_fp32(out, c64, ab, c.rowlen, what, h32)
 
 
@@ -220,7 +220,7 @@ def test_spmv_strided_and_bf16():
     assert np.all(y[:, 1:] == -1.0)
     x0 = np.ascontiguousarray(X[:, 0])
     h32 = O.spmm_host32(RM, c.nr, c.nc, 1, c.rowPtr, c.colIdx, c.values, x0).reshape(-1)
-    short = c.rowlen <= 256
+    short = c.rowlen <= 16
     assert_bitwise(y[:, 0][short], h32[short], "strided spmv")
     d16 = D.to_bf16_device(c.d_B)
     out = run_native(c, m, 1, config=D.make_config(D.VARIANT_SPMV_TILED), btype=D.DSPMM_R_16BF,
@@ -228,3 +228,72 @@ def test_spmv_strided_and_bf16():
     xr = O.round_bf16(c.dense(1))
     h32r = O.spmm_host32(RM, c.nr, c.nc, 1, c.rowPtr, c.colIdx, c.values, xr).reshape(-1)
     assert_bitwise(out[short], h32r[short], "bf16 spmv")
+
+
+@pytest.mark.parametrize("name", ["stencil_20", "c1", "edge", "int_valued"])
+@pytest.mark.parametrize("umax", [64, 128, 256])
+def test_nnz_cached(name, umax):
+    """Column-cached tiles: every row whose columns fit one tile is summed
+    like the host loop (bit for bit); rows that become direct tiles (more
+    distinct columns than the cache, or more nonzeros than a tile) are within
+    the FP32 gate; integer operands are exact everywhere."""
+    c, m = matrix(name)
+    for nv in (2, 8, 16, 64):
+        if nv > c.maxnv:
+            continue
+        h32, c64, ab = c.ref(RM, nv)
+        what = f"{name} cached umax={umax} nv={nv}"
+        note(what)
+        try:
+            out = run_native(c, m, nv, config=D.make_config(D.VARIANT_NNZ_CACHED, tile_nnz=umax))
+        except D.DspmmError as e:
+            assert e.status == D.STATUS_NOT_SUPPORTED, what
+            note(f"{what}: not supported (LDS)")
+            continue
+        fits = np.repeat(c.rowlen <= umax, nv)
+        assert_bitwise(out[fits], h32[fits], what + " (rows that fit a tile)")
+        if name == "int_valued":
+            assert np.array_equal(out, c64.astype(np.float32)), what
+        assert_fp32(out, c64, ab, np.repeat(c.rowlen, nv), what, h32)
+
+
+def test_nnz_cached_bf16_and_autotune():
+    c, m = matrix("stencil_20")
+    nv = 64
+    d16 = D.to_bf16_device(c.d_B)
+    out = run_native(c, m, nv, config=D.make_config(D.VARIANT_NNZ_CACHED, tile_nnz=128),
+                     btype=D.DSPMM_R_16BF, B=d16)
+    br = O.round_bf16(c.dense(nv))
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, br).reshape(-1)
+    assert_bitwise(out, h32r, "stencil_20 cached bf16")
+    # the stencil's neighbouring rows share columns: the autotuner considers the cached kernel
+    tab = m.autotune_table(c.d_B, c.d_C, nv, reps=2)
+    names = {cfg.as_dict()["variant"] for cfg, ms in tab if ms is not None}
+    assert "nnz_cached" in names and "nnz_tiled" in names
+    note(f"stencil_20 nv=64 table: {len(tab)} candidates; best "
+         f"{min((ms, cfg.as_dict()['variant'], cfg.tile_nnz) for cfg, ms in tab if ms is not None)}")
+
+
+def test_learned_selector_resolves_and_runs():
+    """Untuned AUTO on row-major FP32 resolves through the nearest-neighbour
+    selector (not the static nnz-tiled default) and stays within the FP32
+    gate; a fresh handle is used so no autotuned configuration is cached."""
+    picks = {}
+    for name in ("stencil_20", "c1", "zipf_mid", "edge"):
+        c = case(name)
+        m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+        for nv in (1, 8, 32, 64):
+            if nv > c.maxnv:
+                continue
+            picks[(name, nv)] = m.resolve(nv).as_dict()
+            out = run_native(c, m, nv)
+            h32, c64, ab = c.ref(RM, nv)
+            assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), f"selector {name} nv={nv}", h32)
+        # BF16 and column-major keep the static default
+        assert m.resolve(8, btype=D.DSPMM_R_16BF).as_dict()["variant"] == "nnz_tiled"
+        assert m.resolve(8, layout=CM).as_dict()["variant"] == "csr_scalar"
+        m.close()
+    note(f"selector picks: {picks}")
+    assert any(p["variant"] != "nnz_tiled" for p in picks.values())
+    assert all(p["variant"] == "spmv_tiled" or k[1] != 1 or p["variant"] in
+               ("csr_scalar", "csr_vector", "coo_scalar", "coo_vector") for k, p in picks.items())
