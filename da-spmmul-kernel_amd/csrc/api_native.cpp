@@ -160,6 +160,9 @@ void free_matrix(dspmm_matrix_s *A) {
         hipFree(kv.second.d_splits);
         hipFree(kv.second.d_ucols);
         hipFree(kv.second.d_lidx);
+        hipFree(kv.second.d_boff);
+        hipFree(kv.second.d_list);
+        hipFree(kv.second.d_vals);
     }
     hipFree(A->carry);
     delete A;
@@ -219,6 +222,43 @@ dspmm_status_t ensure_rowidx(dspmm_matrix_s *A) {
     return upload(&A->rowIdx, rows.data(), rows.size(), hipMemcpyHostToDevice, nullptr);
 }
 
+bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
+                      const std::vector<float> &va, int nr, int R, std::vector<int> &boff,
+                      std::vector<int2> &list, std::vector<float> &vals) {
+    const int nb = (nr + R - 1) / R;
+    boff.assign((size_t)nb + 1, 0);
+    list.clear();
+    vals.assign(va.size(), 0.f);
+    for (int r = 0; r < nr; r++)                     // sorted rows without repeats
+        for (int p = rp[(size_t)r] + 1; p < rp[(size_t)r + 1]; p++)
+            if (ci[(size_t)p] <= ci[(size_t)p - 1]) return false;
+    int head[8], end[8];
+    for (int b = 0; b < nb; b++) {
+        const int r0 = b * R, nrows = std::min(R, nr - r0);
+        size_t vk = (size_t)rp[(size_t)r0];
+        for (int i = 0; i < nrows; i++) {
+            head[i] = rp[(size_t)r0 + i];
+            end[i] = rp[(size_t)r0 + i + 1];
+        }
+        for (;;) {
+            int c = INT_MAX;
+            for (int i = 0; i < nrows; i++)
+                if (head[i] < end[i]) c = std::min(c, ci[(size_t)head[i]]);
+            if (c == INT_MAX) break;
+            int mask = 0;
+            for (int i = 0; i < nrows; i++)
+                if (head[i] < end[i] && ci[(size_t)head[i]] == c) {
+                    mask |= 1 << i;
+                    vals[vk++] = va[(size_t)head[i]];
+                    head[i]++;
+                }
+            list.push_back(make_int2(c, mask));
+        }
+        boff[(size_t)b + 1] = (int)list.size();
+    }
+    return true;
+}
+
 int cached_smax(int umax) { return std::min(4096, 4 * umax); }
 
 // Host copy of the column indices (device -> host, for plans that need them).
@@ -273,6 +313,38 @@ dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind) {
         if (s != DSPMM_STATUS_SUCCESS) return s;
         build_cached_tiles(A->h_rowPtr, ci, A->nr, A->nc, S, cached_smax(S), tiles, ucols, lidx,
                            reuse);
+    } else if (kind == kPlanRowBlock) {
+        std::vector<int> ci, boff;
+        std::vector<float> va((size_t)A->nnz), rvals;
+        std::vector<int2> list;
+        dspmm_status_t s = host_colidx(A, ci);
+        if (s == DSPMM_STATUS_SUCCESS && A->nnz)
+            s = hip_status(hipMemcpy(va.data(), A->values, sizeof(float) * (size_t)A->nnz,
+                                     hipMemcpyDeviceToHost));
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        NbPlan p;
+        p.S = S;
+        p.usable = build_row_blocks(A->h_rowPtr, ci, va, A->nr, S, boff, list, rvals);
+        if (p.usable) {
+            p.ntiles = (int)boff.size() - 1;
+            p.reuse = list.empty() ? 1.0 : (double)A->nnz / (double)list.size();
+            s = upload(&p.d_boff, boff.data(), boff.size(), hipMemcpyHostToDevice, nullptr);
+            if (s == DSPMM_STATUS_SUCCESS)
+                s = upload(&p.d_list, list.data(), list.size(), hipMemcpyHostToDevice, nullptr);
+            if (s == DSPMM_STATUS_SUCCESS)
+                s = upload(&p.d_vals, rvals.data(), rvals.size(), hipMemcpyHostToDevice, nullptr);
+            if (s != DSPMM_STATUS_SUCCESS) {
+                hipFree(p.d_boff);
+                hipFree(p.d_list);
+                hipFree(p.d_vals);
+                return s;
+            }
+            p.bytes = sizeof(int) * boff.size() + sizeof(int2) * list.size() +
+                      sizeof(float) * rvals.size();
+        }
+        auto res = A->plans.emplace(key, std::move(p));
+        *out = &res.first->second;
+        return DSPMM_STATUS_SUCCESS;
     } else {
         build_tiles(A->h_rowPtr, A->nr, S, tiles, splits, nslots);
     }
@@ -326,6 +398,7 @@ void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *
 constexpr int kSpmvTile = 1024;
 constexpr int kCachedUmaxDefault = 128;
 constexpr double kCachedMinReuse = 1.5;   // sampled nonzeros per distinct column
+constexpr double kRowBlockMinReuse = 1.2;
 
 dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt) {
     (void)A;
@@ -468,6 +541,41 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         if (a.threads != 64 && a.threads != 128 && a.threads != 256) return DSPMM_STATUS_NOT_SUPPORTED;
         if (a.lanes > a.threads) a.threads = a.lanes;
         s = launch_nnz_cached(a, st);
+        if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
+        return s;
+    }
+    case DSPMM_VARIANT_ROW_BLOCKED: {
+        if (layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
+        const int R = cfg.tile_nnz > 0 ? cfg.tile_nnz : 4;
+        if (R != 2 && R != 4 && R != 8) return DSPMM_STATUS_NOT_SUPPORTED;
+        dspmm_config c = cfg;
+        c.tile_nnz = 0;
+        complete_nb(c, nv, ldb, ldc, B, C, bf16);
+        NbPlan *plan = nullptr;
+        dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
+        const Slab sl = make_slab(nv, c.vec);
+        RowBlockLaunch a;
+        a.boff = plan->d_boff;
+        a.list = plan->d_list;
+        a.vals = plan->d_vals;
+        a.rowPtr = A->rowPtr;
+        a.B = B;
+        a.C = C;
+        a.ldb = ldb;
+        a.ldc = ldc;
+        a.nr = A->nr;
+        a.nv = nv;
+        a.nblocks = plan->ntiles;
+        a.rows = R;
+        a.vw = c.vec;
+        a.lanes = std::min(c.lanes, sl.lanes);
+        a.slab_w = a.lanes * c.vec;
+        a.nslabs = (nv + a.slab_w - 1) / a.slab_w;
+        a.unroll = cfg.unroll > 0 ? cfg.unroll : 8;
+        a.bf16 = bf16;
+        s = launch_row_blocked(a, st);
         if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
         return s;
     }
@@ -830,6 +938,16 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
             std::lock_guard<std::mutex> lk(A->mu);
             reuse = column_reuse(A);
         }
+        if (reuse >= kRowBlockMinReuse) {
+            for (int R : {2, 4, 8})
+                for (int u : {4, 8}) {
+                    dspmm_config c{};
+                    c.variant = DSPMM_VARIANT_ROW_BLOCKED;
+                    c.tile_nnz = R;
+                    c.unroll = u;
+                    cand.push_back(c);
+                }
+        }
         if (reuse >= kCachedMinReuse) {
             dspmm_config base{};
             base.variant = DSPMM_VARIANT_NNZ_TILED;
@@ -942,6 +1060,19 @@ dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv, cons
     }
     if (best) *best = best_c;
     if (best_ms) *best_ms = best_t;
+    return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_matrix_set_config(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                       dspmm_dtype_t btype, const dspmm_config *cfg) {
+    if (!A || !cfg || nv < 1 || (layout != DENSE_ROW_MAJOR && layout != DENSE_COL_MAJOR) ||
+        (btype != DSPMM_R_32F && btype != DSPMM_R_16BF) || cfg->variant < DSPMM_VARIANT_AUTO ||
+        cfg->variant > DSPMM_VARIANT_ROW_BLOCKED)
+        return DSPMM_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> lk(A->mu);
+    const auto key = std::make_tuple((int)layout, nv, (int)btype);
+    if (cfg->variant == DSPMM_VARIANT_AUTO) A->tuned.erase(key);
+    else A->tuned[key] = *cfg;
     return DSPMM_STATUS_SUCCESS;
 }
 

@@ -24,6 +24,11 @@ struct NbPlan {
     int *d_ucols = nullptr;
     uint16_t *d_lidx = nullptr;
     double reuse = 0;                 // nonzeros per B row read from L2
+    // row-blocked plans only: per-block list offsets, {column, row mask}, values
+    int *d_boff = nullptr;
+    int2 *d_list = nullptr;
+    float *d_vals = nullptr;
+    bool usable = true;               // false: rows unsorted or with repeated columns
 };
 
 // LDS budget of one nnz-tiled workgroup (keeps two workgroups per CU).
@@ -44,7 +49,15 @@ void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, 
                         int umax, int smax, std::vector<int4> &tiles, std::vector<int> &ucols,
                         std::vector<uint16_t> &lidx, double &reuse);
 
-enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1, kPlanCached = 2 };
+// Row-blocked plan: blocks of R consecutive rows; per block the ascending
+// distinct columns of its rows, each with the mask of rows that have it, and
+// the block's values in (column, row) order.  False if a row is unsorted or
+// repeats a column (the plan then cannot keep every row's nonzero order).
+bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
+                      const std::vector<float> &va, int nr, int R, std::vector<int> &boff,
+                      std::vector<int2> &list, std::vector<float> &vals);
+
+enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1, kPlanCached = 2, kPlanRowBlock = 3 };
 constexpr size_t kMaxCachedLds = 80 * 1024;   // two column-cached workgroups per CU
 
 }  // namespace dspmm

@@ -57,6 +57,25 @@ dspmm_status_t launch_nnz_cached(const CachedLaunch &a, hipStream_t st);
 size_t nnz_cached_lds_bytes(int umax, int smax, int nv, int vw, bool bf16);
 constexpr int kCachedRowsMax = 256;    // rows per cached tile
 
+// One launch of the row-blocked kernel (k_row_blocked): blocks of `rows`
+// consecutive rows; list[boff[b] .. boff[b+1]) = {column, row mask} of block
+// b in ascending column order; vals = the values in that (column, row) order
+// (block b's values start at rowPtr[b * rows]).
+struct RowBlockLaunch {
+    const int *boff = nullptr;
+    const int2 *list = nullptr;
+    const float *vals = nullptr;
+    const int *rowPtr = nullptr;
+    const void *B = nullptr;
+    float *C = nullptr;
+    int64_t ldb = 0, ldc = 0;
+    int nr = 0, nv = 0, nblocks = 0;
+    int rows = 4;                      // R: 2, 4 or 8
+    int vw = 0, lanes = 0, slab_w = 0, nslabs = 1, unroll = 8;
+    bool bf16 = false;
+};
+dspmm_status_t launch_row_blocked(const RowBlockLaunch &a, hipStream_t st);
+
 // Arguments of one single-matrix nnz-tiled launch (plus its split combine).
 struct NbLaunch {
     ItemDesc item{};

@@ -42,6 +42,16 @@ __host__ __device__ inline int wave_lds_ints(int G, int S) {
     return 2 * skew_h(G * S) + G * S + 1;
 }
 
+// Wavefront-level LDS ordering: every wavefront stages into and reads from
+// its own LDS region only, so a workgroup barrier is not needed between
+// staging and use -- the wavefront's LDS writes are made complete and
+// ordered before its reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 struct WaveStage {
     int P0, R0;
     int *s_col;
@@ -161,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_nnz_tiled(ItemDesc d, int ntiles) {
     const int t0 = (b * (kBlock / kWave) + wib) * G;
     const int tcount = t0 < ntiles ? min(G, ntiles - t0) : 0;
     const WaveStage w = stage_tiles(d, t0, tcount, G, smem + wib * wave_lds_ints(G, d.S));
-    __syncthreads();
+    wave_lds_sync();
     process_tiles<TB, VW, L, UNR>(d, t0, tcount, blockIdx.y, w);
 }
 
@@ -201,82 +211,85 @@ __device__ __forceinline__ void with_class(int cls, F &&f) {
 
 // Persistent batched kernel.  segs[xoff[x] .. xoff[x+1]) are XCD x's work
 // segments {item, first unit of the item, unit count, units before this
-// segment in x's list}; xoff[8+x] is x's total unit count.  Workgroups
-// resident on XCD x (blockIdx.x % 8 == x) take consecutive groups of four
-// units of x's list, one per wavefront, until the list is exhausted; every
-// wavefront of a workgroup runs the same number of iterations, so the
-// workgroup barrier that separates LDS staging from its use is uniform.
+// segment in x's list}; xoff[8+1+x] is x's total unit count.  The
+// workgroups resident on XCD x (blockIdx.x % 8 == x) split x's list into
+// contiguous runs, one per workgroup; the four wavefronts take the run's
+// units in turn.  A wavefront finds its first unit's segment by binary
+// search and then walks forward, and it stages into and reads from its own
+// LDS region only, so wavefronts never wait for one another.
 template <typename TB>
 __global__ __launch_bounds__(kBlock) void k_nnz_tiled_batched(const ItemDesc *__restrict__ items,
                                                               const int4 *__restrict__ segs,
                                                               const int *__restrict__ xoff,
                                                               int lds_ints_per_wave) {
     extern __shared__ int smem[];
+    constexpr int W = kBlock / kWave;
     const int wib = (int)threadIdx.x / kWave;
     const int x = blockIdx.x % kNumXcd;
     const int bx = blockIdx.x / kNumXcd;
     const int nbx = ((int)gridDim.x - x + kNumXcd - 1) / kNumXcd;
     const int s_lo = xoff[x], s_hi = xoff[x + 1];
     const int nunits = xoff[kNumXcd + 1 + x];
+    const int run = (nunits + nbx - 1) / nbx;
+    const int u_lo = min(nunits, bx * run), u_hi = min(nunits, u_lo + run);
     int *lds = smem + wib * lds_ints_per_wave;
-    for (int k = bx; k * (kBlock / kWave) < nunits; k += nbx) {
-        const int u = k * (kBlock / kWave) + wib;
-        int item = -1, lu = 0;
-        if (u < nunits) {
-            // last segment whose start <= u (segments sorted by .w)
+    int seg = -1, seg_end = 0;
+    int4 sg = make_int4(0, 0, 0, 0);
+    for (int u = u_lo + wib; u < u_hi; u += W) {
+        if (seg < 0) {                              // last segment whose start <= u
             int lo = s_lo, hi = s_hi - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (segs[mid].w <= u) lo = mid; else hi = mid - 1;
             }
-            const int4 sg = segs[lo];
-            item = sg.x;
-            lu = sg.y + (u - sg.w);
+            seg = lo;
+            sg = segs[seg];
+            seg_end = sg.w + sg.z;
         }
-        ItemDesc d;
-        int t0 = 0, tcount = 0, slab = 0, G = 1;
-        if (item >= 0) {
-            d = items[item];
-            G = kWave >> (d.cls & 7);
-            const int per_slab = (d.ntiles + G - 1) / G;
-            slab = lu / per_slab;
-            t0 = (lu % per_slab) * G;
-            tcount = min(G, d.ntiles - t0);
+        while (u >= seg_end) {                      // walk forward (units only increase)
+            sg = segs[++seg];
+            seg_end = sg.w + sg.z;
         }
-        WaveStage w{};
-        if (item >= 0) w = stage_tiles(d, t0, tcount, G, lds);
-        __syncthreads();
-        if (item >= 0) {
-            with_class<TB>(d.cls, [&](auto vw_c, auto l_c) {
-                constexpr int VW_ = decltype(vw_c)::value, L_ = decltype(l_c)::value;
-                if constexpr (sizeof(TB) == 4 && VW_ == 8) {
-                } else {
-                    process_tiles<TB, VW_, L_, 8>(d, t0, tcount, slab, w);
-                }
-            });
-        }
-        __syncthreads();
+        const int item = sg.x, lu = sg.y + (u - sg.w);
+        const ItemDesc d = items[item];
+        const int G = kWave >> (d.cls & 7);
+        const int per_slab = (d.ntiles + G - 1) / G;
+        const int slab = lu / per_slab;
+        const int t0 = (lu % per_slab) * G;
+        const int tcount = min(G, d.ntiles - t0);
+        const WaveStage w = stage_tiles(d, t0, tcount, G, lds);
+        wave_lds_sync();
+        with_class<TB>(d.cls, [&](auto vw_c, auto l_c) {
+            constexpr int VW_ = decltype(vw_c)::value, L_ = decltype(l_c)::value;
+            if constexpr (sizeof(TB) == 4 && VW_ == 8) {
+            } else {
+                process_tiles<TB, VW_, L_, 8>(d, t0, tcount, slab, w);
+            }
+        });
+        wave_lds_sync();                            // LDS is restaged for the next unit
     }
 }
 
 // ---- split-row combine ------------------------------------------------------
-// One workgroup per split row.  Canonical order, a function of (nv, nchunks)
-// only: Lc = min(256, next power of two >= nv) threads cover the columns,
-// the chunks are cut into P = 256 / Lc contiguous parts, each part is summed
-// in chunk order by one thread per column, and the P part sums are added in
-// part order.  Single-matrix and batched launches share this function, so
-// their results agree bit for bit.  A thread loads up to 16 chunk values
-// before adding them, so its loads overlap.
-__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp, float *lds) {
+// One wavefront per split row, four per workgroup.  Canonical order, a
+// function of (nv, nchunks) only: Lw = min(64, next power of two >= nv)
+// lanes cover a block of columns (column blocks of Lw in turn when nv > 64),
+// the chunks are cut into P = 64 / Lw contiguous parts, each part is summed
+// in chunk order by one lane per column, and the first part's lanes add the
+// P part sums in part order (read across lanes with ds_bpermute).
+// Single-matrix and batched launches share this function, so their results
+// agree bit for bit.  A lane loads up to 16 chunk values before adding them,
+// so its loads overlap.
+__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp) {
     const int row = sp.y, slot0 = sp.z, nch = sp.w;
-    const int t = threadIdx.x;
-    int Lc = 1;
-    while (Lc < d.nv && Lc < kBlock) Lc <<= 1;
-    const int P = kBlock / Lc;
-    const int q = t / Lc, jl = t % Lc;
+    const int ln = lane_id();
+    int Lw = 1;
+    while (Lw < d.nv && Lw < kWave) Lw <<= 1;
+    const int P = kWave / Lw;
+    const int q = ln / Lw, jl = ln % Lw;
     const int per = (nch + P - 1) / P;
     const int c0 = min(nch, q * per), c1 = min(nch, c0 + per);
-    for (int jb = 0; jb < d.nv; jb += Lc) {          // uniform loop
+    for (int jb = 0; jb < d.nv; jb += Lw) {          // wave-uniform loop
         const int j = jb + jl;
         float s = 0.f;
         if (j < d.nv) {
@@ -290,13 +303,9 @@ __device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp, float 
                     if (c + u < c1) s = s + v[u];
             }
         }
-        if (P > 1) {
-            lds[t] = s;
-            __syncthreads();
-            if (q == 0) {
-                for (int qq = 1; qq < P; qq++) s = s + lds[qq * Lc + jl];
-            }
-            __syncthreads();
+        for (int qq = 1; qq < P; qq++) {             // wave-uniform; every lane takes part
+            const float o = __shfl(s, qq * Lw + jl, kWave);
+            if (q == 0) s = s + o;
         }
         if (q == 0 && j < d.nv) d.C[(int64_t)row * d.ldc + j] = s;
     }
@@ -304,24 +313,25 @@ __device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp, float 
 
 __global__ __launch_bounds__(kBlock) void k_split_combine(ItemDesc d, const int4 *__restrict__ splits,
                                                           int nsplits) {
-    __shared__ float lds[kBlock];
-    combine_split(d, splits[blockIdx.x], lds);
+    const int w = blockIdx.x * (kBlock / kWave) + (int)threadIdx.x / kWave;
+    if (w >= nsplits) return;                       // whole wavefronts leave; no barrier follows
+    combine_split(d, splits[w]);
 }
 
-// Batched: workgroup b finds its item by binary search in the prefix sums
+// Batched: wavefront w finds its item by binary search in the prefix sums
 // of the items' split counts.
 __global__ __launch_bounds__(kBlock) void k_split_combine_batched(const ItemDesc *__restrict__ items,
                                                                   const int *__restrict__ prefix,
-                                                                  int count) {
-    __shared__ float lds[kBlock];
-    const int b = blockIdx.x;
-    int lo = 0, hi = count - 1;                     // last item with prefix[i] <= b
+                                                                  int count, int nsplits) {
+    const int w = blockIdx.x * (kBlock / kWave) + (int)threadIdx.x / kWave;
+    if (w >= nsplits) return;
+    int lo = 0, hi = count - 1;                     // last item with prefix[i] <= w
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+        if (prefix[mid] <= w) lo = mid; else hi = mid - 1;
     }
     const ItemDesc &d = items[lo];
-    combine_split(d, d.splits[b - prefix[lo]], lds);
+    combine_split(d, d.splits[w - prefix[lo]]);
 }
 
 // ---- SpMV (nv == 1) -----------------------------------------------------------
@@ -563,6 +573,67 @@ __global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
     }
 }
 
+// ---- row-blocked SpMM (row-major) -------------------------------------------
+// For matrices whose neighbouring rows share columns.  A host plan groups
+// the rows in blocks of R consecutive rows and lists, per block, the
+// distinct columns of those rows in ascending order, each with a mask of
+// the block's rows that have it ({col, mask}), and the block's values in
+// that (column, row) order.  A group of L lanes x VW columns owns a block:
+// it loads each listed B row once and adds it into the accumulators of the
+// rows in its mask.  Rows are sorted, so each row still sums its nonzeros
+// in nonzero order with a separately rounded multiply and add -- bit-exact
+// with the host loop -- while the B rows a block's rows share are gathered
+// once instead of once per row (a 27-point stencil: 9 (R + 2) gathers for
+// 27 R nonzeros).
+template <typename TB, int VW, int L, int R, int UNR>
+__global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
+    constexpr int G = kBlock / L;
+    const int t = threadIdx.x;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x) * G + t / L;
+    const int col0 = blockIdx.y * a.slab_w + (t % L) * VW;
+    if (blk >= a.nblocks || col0 >= a.nv) return;
+    const int r0 = blk * R;
+    const int u0 = a.boff[blk], u1 = a.boff[blk + 1];
+    const int vb = a.rowPtr[r0];
+    const int ve = a.rowPtr[min(a.nr, r0 + R)];
+    const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B);
+    float acc[R][VW];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+#pragma unroll
+        for (int k = 0; k < VW; k++) acc[i][k] = 0.f;
+    int vk = vb;                                     // next value of the block
+    for (int u = u0; u < u1; u += UNR) {
+        int2 cm[UNR];
+        float bv[UNR][VW];
+#pragma unroll
+        for (int x = 0; x < UNR; x++) cm[x] = (u + x < u1) ? a.list[u + x] : make_int2(0, 0);
+#pragma unroll
+        for (int x = 0; x < UNR; x++)
+            if (u + x < u1) load_b<TB, VW>(bv[x], B + (int64_t)cm[x].x * a.ldb + col0);
+#pragma unroll
+        for (int x = 0; x < UNR; x++) {
+            if (u + x < u1) {
+                const int mask = cm[x].y;
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    const bool has = (mask >> i) & 1;
+                    const float v = a.vals[min(vk, ve - 1)];
+                    vk += has ? 1 : 0;
+#pragma unroll
+                    for (int k = 0; k < VW; k++) {
+                        const float sum = acc[i][k] + v * bv[x][k];
+                        acc[i][k] = has ? sum : acc[i][k];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        if (r0 + i < a.nr) store_vec_nt<VW>(a.C + (int64_t)(r0 + i) * a.ldc + col0, acc[i]);
+}
+
 __global__ void k_fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
@@ -675,8 +746,8 @@ dspmm_status_t launch_nnz_tiled(const NbLaunch &a, hipStream_t st) {
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine, dim3(a.nsplits), dim3(kBlock), 0, st, a.item, a.splits,
-                           a.nsplits);
+        hipLaunchKernelGGL(k_split_combine, dim3(div_up(a.nsplits, kBlock / kWave)), dim3(kBlock), 0,
+                           st, a.item, a.splits, a.nsplits);
         s = launch_status();
     }
     return s;
@@ -696,8 +767,8 @@ dspmm_status_t launch_nnz_tiled_batched(const BatchLaunch &a, hipStream_t st) {
     }
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine_batched, dim3(a.nsplits), dim3(kBlock), 0, st, a.items,
-                           a.split_prefix, a.count);
+        hipLaunchKernelGGL(k_split_combine_batched, dim3(div_up(a.nsplits, kBlock / kWave)),
+                           dim3(kBlock), 0, st, a.items, a.split_prefix, a.count, a.nsplits);
         return launch_status();
     }
     return DSPMM_STATUS_SUCCESS;
@@ -783,6 +854,58 @@ dspmm_status_t launch_nnz_cached(const CachedLaunch &a, hipStream_t st) {
     }
 }
 
+namespace {
+template <typename TB, int VW, int L, int R>
+dspmm_status_t launch_rb_r(const RowBlockLaunch &a, hipStream_t st) {
+    constexpr int G = kBlock / L;
+    const dim3 grid(div_up(a.nblocks, G), a.nslabs);
+    if (a.unroll <= 4) hipLaunchKernelGGL((k_row_blocked<TB, VW, L, R, 4>), grid, dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((k_row_blocked<TB, VW, L, R, 8>), grid, dim3(kBlock), 0, st, a);
+    return launch_status();
+}
+template <typename TB, int VW, int L>
+dspmm_status_t launch_rb_l(const RowBlockLaunch &a, hipStream_t st) {
+    switch (a.rows) {
+    case 2: return launch_rb_r<TB, VW, L, 2>(a, st);
+    case 4: return launch_rb_r<TB, VW, L, 4>(a, st);
+    case 8: return launch_rb_r<TB, VW, L, 8>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+template <typename TB, int VW>
+dspmm_status_t launch_rb_v(const RowBlockLaunch &a, hipStream_t st) {
+    switch (a.lanes) {
+    case 1: return launch_rb_l<TB, VW, 1>(a, st);
+    case 2: return launch_rb_l<TB, VW, 2>(a, st);
+    case 4: return launch_rb_l<TB, VW, 4>(a, st);
+    case 8: return launch_rb_l<TB, VW, 8>(a, st);
+    case 16: return launch_rb_l<TB, VW, 16>(a, st);
+    case 32: return launch_rb_l<TB, VW, 32>(a, st);
+    case 64: return launch_rb_l<TB, VW, 64>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+}  // namespace
+
+dspmm_status_t launch_row_blocked(const RowBlockLaunch &a, hipStream_t st) {
+    if (a.nblocks == 0) return DSPMM_STATUS_SUCCESS;
+    if (a.bf16) {
+        switch (a.vw) {
+        case 8: return launch_rb_v<uint16_t, 8>(a, st);
+        case 4: return launch_rb_v<uint16_t, 4>(a, st);
+        case 2: return launch_rb_v<uint16_t, 2>(a, st);
+        case 1: return launch_rb_v<uint16_t, 1>(a, st);
+        default: return DSPMM_STATUS_NOT_SUPPORTED;
+        }
+    }
+    switch (a.vw) {
+    case 4: return launch_rb_v<float, 4>(a, st);
+    case 2: return launch_rb_v<float, 2>(a, st);
+    case 1: return launch_rb_v<float, 1>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+
 bool spmv_tile_ok(int T) { return T == 512 || T == 1024 || T == 2048 || T == 4096; }
 
 dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
@@ -791,8 +914,8 @@ dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine, dim3(a.nsplits), dim3(kBlock), 0, st, a.item, a.splits,
-                           a.nsplits);
+        hipLaunchKernelGGL(k_split_combine, dim3(div_up(a.nsplits, kBlock / kWave)), dim3(kBlock), 0,
+                           st, a.item, a.splits, a.nsplits);
         s = launch_status();
     }
     return s;

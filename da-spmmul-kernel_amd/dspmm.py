@@ -48,8 +48,10 @@ VARIANT_AUTO, VARIANT_NNZ_TILED = 0, 1
 VARIANT_CSR_SCALAR, VARIANT_CSR_VECTOR, VARIANT_COO_SCALAR, VARIANT_COO_VECTOR = 2, 3, 4, 5
 VARIANT_SPMV_TILED = 6
 VARIANT_NNZ_CACHED = 7
+VARIANT_ROW_BLOCKED = 8
 VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
-                 4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled", 7: "nnz_cached"}
+                 4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled", 7: "nnz_cached",
+                 8: "row_blocked"}
 MATRIX_WITH_COO = 0x1
 
 STATUS_SUCCESS = 0
@@ -189,6 +191,7 @@ def _declare(lib):
         "dspmm_spmm_ex_resolve": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(Config)], _ST),
         "dspmm_autotune": ([_P, _I, _I, _P, _I64, _I, _P, _I64, _I, C.POINTER(Config),
                             C.POINTER(C.c_float), _P], _ST),
+        "dspmm_matrix_set_config": ([_P, _I, _I, _I, C.POINTER(Config)], _ST),
         "dspmm_autotune_table": ([_P, _I, _I, _P, _I64, _I, _P, _I64, _I, C.POINTER(Config),
                                   C.POINTER(C.c_float), _I, C.POINTER(C.c_int), _P], _ST),
         "dspmm_benchmark_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _I, _I,
@@ -459,6 +462,11 @@ class Matrix:
         check(lib().dspmm_autotune(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out), ldc,
                                    reps, C.byref(best), C.byref(ms), stream), "autotune")
         return best, ms.value
+
+    def set_config(self, nv, config, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F):
+        """Pins `config` as the tuned configuration for (layout, nv, btype)."""
+        check(lib().dspmm_matrix_set_config(self.handle, layout, nv, btype, C.byref(config)),
+              "matrix_set_config")
 
     def autotune_table(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, reps=5,
                        ldb=None, ldc=None, stream=None):

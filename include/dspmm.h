@@ -246,7 +246,7 @@ typedef enum dspmm_variant {
     DSPMM_VARIANT_SPMV_TILED = 6,/* nv == 1 only: workgroup per nnz tile, products
                                     in LDS, per-row sums (row-exact), tree for
                                     long rows; tile_nnz = tile size */
-    DSPMM_VARIANT_NNZ_CACHED = 7 /* row-major, lanes * vec >= nv: tiles of whole
+    DSPMM_VARIANT_NNZ_CACHED = 7,/* row-major, lanes * vec >= nv: tiles of whole
                                     rows whose distinct columns' B rows are
                                     staged in LDS once per tile (matrices whose
                                     neighbouring rows share columns); rows are
@@ -254,6 +254,12 @@ typedef enum dspmm_variant {
                                     tile_nnz = distinct columns per tile,
                                     unroll = workgroup size (64/128/256,
                                     0 = from the plan's rows per tile) */
+    DSPMM_VARIANT_ROW_BLOCKED = 8  /* row-major, sorted rows without repeated
+                                    columns: a lane group owns tile_nnz = 2, 4
+                                    or 8 consecutive rows and loads each B row
+                                    their columns share once; rows are summed
+                                    like the host loop (row-exact); unroll =
+                                    B rows in flight (4 or 8) */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {
@@ -291,6 +297,11 @@ DSPMM_API dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, in
                                         float *C, int64_t ldc, int reps,
                                         dspmm_config *best, float *best_ms,
                                         dspmm_stream_t stream);
+/* Stores cfg as the tuned configuration for (layout, nv, btype), as
+ * dspmm_autotune would (e.g. one saved from an earlier process); AUTO calls
+ * and batches built afterwards use it.  cfg->variant AUTO clears it. */
+DSPMM_API dspmm_status_t dspmm_matrix_set_config(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                                 dspmm_dtype_t btype, const dspmm_config *cfg);
 /* The autotuner's whole table: every candidate it would time for these
  * operands and its median ms over `reps` calls (negative: not supported).
  * With capacity == 0 only *count is set; a capacity below *count gives

@@ -297,3 +297,42 @@ def test_learned_selector_resolves_and_runs():
     assert any(p["variant"] != "nnz_tiled" for p in picks.values())
     assert all(p["variant"] == "spmv_tiled" or k[1] != 1 or p["variant"] in
                ("csr_scalar", "csr_vector", "coo_scalar", "coo_vector") for k, p in picks.items())
+
+
+@pytest.mark.parametrize("name", ["stencil_20", "c1", "edge", "int_valued", "zipf_mid"])
+@pytest.mark.parametrize("R,unroll", [(2, 4), (4, 8), (8, 8)])
+def test_row_blocked(name, R, unroll):
+    """Row blocks: every row sums its nonzeros in nonzero order -- bit-exact
+    with the host loop on all rows (no splits), FP32 and BF16-stored B."""
+    c, m = matrix(name)
+    cfg = D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=R, unroll=unroll)
+    for nv in (1, 3, 8, 64):
+        if nv > c.maxnv:
+            continue
+        h32, c64, ab = c.ref(RM, nv)
+        what = f"{name} row-blocked R={R} nv={nv}"
+        note(what)
+        out = run_native(c, m, nv, config=cfg)
+        assert_bitwise(out, h32, what)
+    nv = min(64, c.maxnv)
+    d16 = D.to_bf16_device(c.d_B)
+    out = run_native(c, m, nv, config=cfg, btype=D.DSPMM_R_16BF, B=d16)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(c.dense(nv))).reshape(-1)
+    assert_bitwise(out, h32r, f"{name} row-blocked R={R} bf16")
+
+
+def test_row_blocked_rejects_repeated_columns():
+    rp = np.array([0, 3, 4], np.int32)
+    ci = np.array([1, 1, 2, 0], np.int32)
+    v = np.ones(4, np.float32)
+    m = D.Matrix(2, 3, rp, ci, v)
+    d_B = D.DeviceArray.from_numpy(np.ones((3, 4), np.float32))
+    d_C = D.DeviceArray.zeros((2, 4))
+    with pytest.raises(D.DspmmError) as e:
+        m.spmm(d_B, d_C, 4, config=D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4))
+    assert e.value.status == D.STATUS_NOT_SUPPORTED
+    m.spmm(d_B, d_C, 4)                               # AUTO still runs (another kernel)
+    D.synchronize()
+    assert np.array_equal(d_C.numpy(), np.array([[3] * 4, [1] * 4], np.float32))
+    m.close()
