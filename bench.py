@@ -223,8 +223,10 @@ def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
     requests from the c4 pool (`count` matrices, each with its N) and 5% from
     `n_large` large power-law matrices (M=K=2e6, ~40M nnz, N=64).  B is
     generated on the device per request (seeded by the request id).  The
-    same stream is replayed against (i) the continuous-batching server and
-    (ii) a plain server that launches each request on its own; then every
+    same stream is replayed against (i) the continuous-batching server,
+    (ii) a plain server that launches each request on its own with the
+    batch's configuration, and (iii) one that uses each matrix's own AUTO
+    (selected) configuration; then every
     request is submitted at once to measure saturation throughput, and once
     more with the large requests left out (the large ones are bound by their
     own HBM traffic, which batching does not reduce)."""
@@ -241,7 +243,7 @@ def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
     allm = pool + large
     res = {"requests": n_req, "rate_offered": rate, "large_fraction": float(np.mean(cls)),
            "pool": count, "large_pool": n_large}
-    for plain in (False, True):
+    for plain in (0, 1, 2):
         srv = D.Server(cache_bytes=200 << 30, operand_bytes=64 << 30, max_inflight=3,
                        max_batch=256, plain=plain)
         mids = [srv.add_matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values, preload=True)
@@ -250,7 +252,7 @@ def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
                 for i, (_, nv) in enumerate(allm)]
         srv.replay(warm, [0.0] * len(warm))
         srv.poll()
-        key = "plain" if plain else "batched"
+        key = ("batched", "plain", "plain_own_best")[plain]
         small = [k for k in range(n_req) if pick[k] < count]
         for mode, ids, times in (("poisson", range(n_req), at),
                                  ("saturation", range(n_req), np.zeros(n_req)),
@@ -276,9 +278,11 @@ def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
             log(f"serving {key} {mode}: {r}")
         res[f"{key}_stats"] = {k: v for k, v in srv.stats().items()}
         srv.close()
+    # against the faster plain server: the batch's configuration per request,
+    # or each matrix's own AUTO (selected) configuration
     for mode in ("saturation", "saturation_small"):
-        res[f"throughput_ratio_{mode}"] = (res[f"batched_{mode}"]["req_per_s"] /
-                                           res[f"plain_{mode}"]["req_per_s"])
+        plain = max(res[f"plain_{mode}"]["req_per_s"], res[f"plain_own_best_{mode}"]["req_per_s"])
+        res[f"throughput_ratio_{mode}"] = res[f"batched_{mode}"]["req_per_s"] / plain
     return res
 
 

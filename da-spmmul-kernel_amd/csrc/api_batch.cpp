@@ -33,13 +33,11 @@ dspmm_status_t build_schedule(const dspmm_batch_item *items, int count, dspmm_dt
             (A->nnz > 0 && !it.B))
             return DSPMM_STATUS_INVALID_VALUE;
         std::lock_guard<std::mutex> lk(A->mu);
-        // tuned configuration if cached, else the static default: the batch
-        // schedule's efficiency is not what single launches were measured on
-        dspmm_config c = resolve(A, DENSE_ROW_MAJOR, it.nv, btype, nullptr, false);
-        if (c.variant != DSPMM_VARIANT_NNZ_TILED) {
-            c = dspmm_config{};
-            c.variant = DSPMM_VARIANT_NNZ_TILED;
-        }
+        // The static nnz-tiled default: configurations tuned on single
+        // launches (where one matrix has the whole GPU and its caches) make
+        // the batch slower, not faster.
+        dspmm_config c{};
+        c.variant = DSPMM_VARIANT_NNZ_TILED;
         complete_nb(c, it.nv, it.ldb, it.ldc, it.B, it.C, bf16);
         if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) {
             c.tile_nnz = 16;

@@ -228,14 +228,13 @@ bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
     const int nb = (nr + R - 1) / R;
     boff.assign((size_t)nb + 1, 0);
     list.clear();
-    vals.assign(va.size(), 0.f);
+    vals.clear();
     for (int r = 0; r < nr; r++)                     // sorted rows without repeats
         for (int p = rp[(size_t)r] + 1; p < rp[(size_t)r + 1]; p++)
             if (ci[(size_t)p] <= ci[(size_t)p - 1]) return false;
     int head[8], end[8];
     for (int b = 0; b < nb; b++) {
         const int r0 = b * R, nrows = std::min(R, nr - r0);
-        size_t vk = (size_t)rp[(size_t)r0];
         for (int i = 0; i < nrows; i++) {
             head[i] = rp[(size_t)r0 + i];
             end[i] = rp[(size_t)r0 + i + 1];
@@ -246,10 +245,12 @@ bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
                 if (head[i] < end[i]) c = std::min(c, ci[(size_t)head[i]]);
             if (c == INT_MAX) break;
             int mask = 0;
+            const size_t v0 = vals.size();
+            vals.resize(v0 + (size_t)R, 0.f);        // R values per listed column, 0 where absent
             for (int i = 0; i < nrows; i++)
                 if (head[i] < end[i] && ci[(size_t)head[i]] == c) {
                     mask |= 1 << i;
-                    vals[vk++] = va[(size_t)head[i]];
+                    vals[v0 + (size_t)i] = va[(size_t)head[i]];
                     head[i]++;
                 }
             list.push_back(make_int2(c, mask));

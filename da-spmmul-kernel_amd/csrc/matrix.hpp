@@ -51,8 +51,9 @@ void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, 
 
 // Row-blocked plan: blocks of R consecutive rows; per block the ascending
 // distinct columns of its rows, each with the mask of rows that have it, and
-// the block's values in (column, row) order.  False if a row is unsorted or
-// repeats a column (the plan then cannot keep every row's nonzero order).
+// per listed column R values (the rows' values, 0 where the mask bit is
+// clear).  False if a row is unsorted or repeats a column (the plan then
+// cannot keep every row's nonzero order).
 bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
                       const std::vector<float> &va, int nr, int R, std::vector<int> &boff,
                       std::vector<int2> &list, std::vector<float> &vals);

@@ -59,8 +59,8 @@ constexpr int kCachedRowsMax = 256;    // rows per cached tile
 
 // One launch of the row-blocked kernel (k_row_blocked): blocks of `rows`
 // consecutive rows; list[boff[b] .. boff[b+1]) = {column, row mask} of block
-// b in ascending column order; vals = the values in that (column, row) order
-// (block b's values start at rowPtr[b * rows]).
+// b in ascending column order; vals[u * rows + i] = row i's value at listed
+// column u (0 where the mask bit is clear).
 struct RowBlockLaunch {
     const int *boff = nullptr;
     const int2 *list = nullptr;

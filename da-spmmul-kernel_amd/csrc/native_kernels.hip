@@ -211,12 +211,14 @@ __device__ __forceinline__ void with_class(int cls, F &&f) {
 
 // Persistent batched kernel.  segs[xoff[x] .. xoff[x+1]) are XCD x's work
 // segments {item, first unit of the item, unit count, units before this
-// segment in x's list}; xoff[8+1+x] is x's total unit count.  The
-// workgroups resident on XCD x (blockIdx.x % 8 == x) split x's list into
-// contiguous runs, one per workgroup; the four wavefronts take the run's
-// units in turn.  A wavefront finds its first unit's segment by binary
-// search and then walks forward, and it stages into and reads from its own
-// LDS region only, so wavefronts never wait for one another.
+// segment in x's list}, in item order; xoff[8+1+x] is x's total unit count.
+// The workgroups resident on XCD x (blockIdx.x % 8 == x) take x's units in
+// list order, four consecutive units (one per wavefront) per workgroup and
+// step, so at any moment an XCD works on a narrow window of its list (few
+// matrices, whose B rows then stay in that XCD's L2).  A wavefront finds its
+// first unit's segment by binary search and then walks forward; it stages
+// into and reads from its own LDS region only, so wavefronts never wait for
+// one another.
 template <typename TB>
 __global__ __launch_bounds__(kBlock) void k_nnz_tiled_batched(const ItemDesc *__restrict__ items,
                                                               const int4 *__restrict__ segs,
@@ -230,12 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_nnz_tiled_batched(const ItemDesc *__
     const int nbx = ((int)gridDim.x - x + kNumXcd - 1) / kNumXcd;
     const int s_lo = xoff[x], s_hi = xoff[x + 1];
     const int nunits = xoff[kNumXcd + 1 + x];
-    const int run = (nunits + nbx - 1) / nbx;
-    const int u_lo = min(nunits, bx * run), u_hi = min(nunits, u_lo + run);
     int *lds = smem + wib * lds_ints_per_wave;
     int seg = -1, seg_end = 0;
     int4 sg = make_int4(0, 0, 0, 0);
-    for (int u = u_lo + wib; u < u_hi; u += W) {
+    for (int u = bx * W + wib; u < nunits; u += nbx * W) {
         if (seg < 0) {                              // last segment whose start <= u
             int lo = s_lo, hi = s_hi - 1;
             while (lo < hi) {
@@ -585,6 +585,22 @@ __global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
 // with the host loop -- while the B rows a block's rows share are gathered
 // once instead of once per row (a 27-point stencil: 9 (R + 2) gathers for
 // 27 R nonzeros).
+template <int R>
+__device__ __forceinline__ void load_row_vals(float (&v)[R], const float *p) {
+    if constexpr (R == 8) {
+        const float4 a = reinterpret_cast<const float4 *>(p)[0];
+        const float4 b = reinterpret_cast<const float4 *>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else if constexpr (R == 4) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+        const float2 a = *reinterpret_cast<const float2 *>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
+}
+
 template <typename TB, int VW, int L, int R, int UNR>
 __global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
     constexpr int G = kBlock / L;
@@ -594,35 +610,35 @@ __global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
     if (blk >= a.nblocks || col0 >= a.nv) return;
     const int r0 = blk * R;
     const int u0 = a.boff[blk], u1 = a.boff[blk + 1];
-    const int vb = a.rowPtr[r0];
-    const int ve = a.rowPtr[min(a.nr, r0 + R)];
     const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B);
     float acc[R][VW];
 #pragma unroll
     for (int i = 0; i < R; i++)
 #pragma unroll
         for (int k = 0; k < VW; k++) acc[i][k] = 0.f;
-    int vk = vb;                                     // next value of the block
     for (int u = u0; u < u1; u += UNR) {
         int2 cm[UNR];
-        float bv[UNR][VW];
+        float bv[UNR][VW], v[UNR][R];
 #pragma unroll
-        for (int x = 0; x < UNR; x++) cm[x] = (u + x < u1) ? a.list[u + x] : make_int2(0, 0);
+        for (int x = 0; x < UNR; x++) {
+            cm[x] = make_int2(0, 0);
+            if (u + x < u1) {
+                cm[x] = a.list[u + x];
+                load_row_vals<R>(v[x], a.vals + (int64_t)(u + x) * R);
+            }
+        }
 #pragma unroll
         for (int x = 0; x < UNR; x++)
             if (u + x < u1) load_b<TB, VW>(bv[x], B + (int64_t)cm[x].x * a.ldb + col0);
 #pragma unroll
         for (int x = 0; x < UNR; x++) {
             if (u + x < u1) {
-                const int mask = cm[x].y;
 #pragma unroll
                 for (int i = 0; i < R; i++) {
-                    const bool has = (mask >> i) & 1;
-                    const float v = a.vals[min(vk, ve - 1)];
-                    vk += has ? 1 : 0;
+                    const bool has = (cm[x].y >> i) & 1;
 #pragma unroll
                     for (int k = 0; k < VW; k++) {
-                        const float sum = acc[i][k] + v * bv[x][k];
+                        const float sum = acc[i][k] + v[x][i] * bv[x][k];
                         acc[i][k] = has ? sum : acc[i][k];
                     }
                 }

@@ -38,6 +38,10 @@ double now_s() {
 
 size_t round_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
+// Matrices with at least this many nonzeros fill the GPU alone; the batching
+// server launches their requests one at a time.
+constexpr int64_t kServeAloneNnz = 16 << 20;
+
 // First-fit allocator over one device region; offsets aligned to 256 B.
 class Arena {
   public:
@@ -185,15 +189,30 @@ struct dspmm_server_s {
 
     // ---- admission -------------------------------------------------------------
     // Reserves arena space for a queued job; false if it does not fit now.
+    // A request on a matrix that fills the GPU by itself is launched alone,
+    // with the matrix's own AUTO configuration: batching cannot make it
+    // faster, and the batch's static configuration makes it slower.
+    bool runs_alone(const SMat &m) const {
+        return cfg.plain == 0 && (int64_t)m.nnz >= (cfg.alone_nnz > 0 ? cfg.alone_nnz : kServeAloneNnz);
+    }
+
     bool reserve(Job &j, SMat &m) {
         const dspmm_request &r = j.r;
         j.szB = r.B ? 0 : (size_t)m.nc * (size_t)r.nv * elem_b();
         j.szC = r.C ? 0 : (size_t)m.nr * (size_t)r.nv * sizeof(float);
         size_t carry = 0;
-        dspmm_batch_item it{m.h, r.nv, reinterpret_cast<const void *>(256), r.nv,
-                            reinterpret_cast<float *>(256), r.nv};
-        BatchSchedule one;
-        if (build_schedule(&it, 1, cfg.btype, one) == DSPMM_STATUS_SUCCESS) carry = one.carry_bytes[0];
+        if (runs_alone(m)) {
+            // its own AUTO configuration's workspace (slots run concurrently,
+            // so the handle's internal buffer cannot be shared)
+            (void)dspmm_spmm_ex_workspace_size(m.h, DENSE_ROW_MAJOR, r.nv, cfg.btype, nullptr,
+                                               &carry);
+        } else {
+            dspmm_batch_item it{m.h, r.nv, reinterpret_cast<const void *>(256), r.nv,
+                                reinterpret_cast<float *>(256), r.nv};
+            BatchSchedule one;
+            if (build_schedule(&it, 1, cfg.btype, one) == DSPMM_STATUS_SUCCESS)
+                carry = one.carry_bytes[0];
+        }
         j.szK = carry;
         bool okB = true, okC = true, okK = true;
         if (j.szB) okB = arena.alloc(j.szB, &j.offB);
@@ -214,9 +233,15 @@ struct dspmm_server_s {
     bool launch(Slot &sl) {
         std::vector<Job> jobs;
         const int cap = cfg.plain ? 1 : cfg.max_batch;
+        bool alone = false;
         while (!queue.empty() && (int)jobs.size() < cap) {
             Job &j = queue.front();
             SMat &m = mats[(size_t)j.r.matrix];
+            alone = runs_alone(m);
+            if (alone && !jobs.empty()) {            // it starts its own launch next time
+                alone = false;
+                break;
+            }
             if (!ensure_resident(m)) break;
             m.pins++;
             if (!reserve(j, m)) {
@@ -225,6 +250,7 @@ struct dspmm_server_s {
             }
             jobs.push_back(j);
             queue.pop_front();
+            if (alone) break;
         }
         if (jobs.empty()) {
             // Nothing in flight can free cache or arena space: the head request
@@ -258,7 +284,20 @@ struct dspmm_server_s {
         dspmm_status_t st = build_schedule(items.data(), (int)items.size(), cfg.btype, sch);
         if (st == DSPMM_STATUS_SUCCESS) {
             for (size_t i = 0; i < jobs.size(); i++) sch.items[i].carry = jobs[i].carry;
-            if (cfg.plain) {
+            if (alone) {
+                const Job &j = jobs[0];
+                st = dspmm_spmm_ex(items[0].A, DENSE_ROW_MAJOR, j.r.nv, j.B, j.r.nv, cfg.btype, j.C,
+                                   j.r.nv, nullptr, j.carry, j.szK,
+                                   reinterpret_cast<dspmm_stream_t>(sl.st));
+            } else if (cfg.plain == 2) {
+                // the matrix's own AUTO configuration and workspace (requests
+                // run in order on one stream, so the handle's buffer is not shared
+                // by two launches at once)
+                const Job &j = jobs[0];
+                st = dspmm_spmm_ex(items[0].A, DENSE_ROW_MAJOR, j.r.nv, j.B, j.r.nv, cfg.btype, j.C,
+                                   j.r.nv, nullptr, nullptr, 0,
+                                   reinterpret_cast<dspmm_stream_t>(sl.st));
+            } else if (cfg.plain) {
                 const Job &j = jobs[0];
                 st = dspmm_spmm_ex(items[0].A, DENSE_ROW_MAJOR, j.r.nv, j.B, j.r.nv, cfg.btype, j.C,
                                    j.r.nv, &sch.cfg[0], j.carry, j.szK,
@@ -382,6 +421,7 @@ double dspmm_server_now(void) { return now_s(); }
 dspmm_status_t dspmm_server_create(dspmm_server_t *out, const dspmm_server_config *cfg) {
     if (!out || !cfg || cfg->max_inflight < 1 || cfg->max_batch < 1) return DSPMM_STATUS_INVALID_VALUE;
     if (cfg->btype != DSPMM_R_32F && cfg->btype != DSPMM_R_16BF) return DSPMM_STATUS_INVALID_VALUE;
+    if (cfg->plain < 0 || cfg->plain > 2 || cfg->alone_nnz < 0) return DSPMM_STATUS_INVALID_VALUE;
     *out = nullptr;
     auto *s = new dspmm_server_s;
     s->cfg = *cfg;

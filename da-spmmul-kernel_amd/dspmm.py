@@ -111,7 +111,7 @@ class BatchInfo(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("cache_bytes", C.c_size_t), ("operand_bytes", C.c_size_t),
                 ("max_inflight", C.c_int), ("max_batch", C.c_int), ("plain", C.c_int),
-                ("btype", C.c_int)]
+                ("btype", C.c_int), ("alone_nnz", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -591,8 +591,9 @@ class Server:
     admission and continuous batching, run by a library thread."""
 
     def __init__(self, cache_bytes=200 << 30, operand_bytes=16 << 30, max_inflight=3,
-                 max_batch=128, plain=False, btype=DSPMM_R_32F):
-        cfg = ServerConfig(cache_bytes, operand_bytes, max_inflight, max_batch, int(plain), btype)
+                 max_batch=128, plain=False, btype=DSPMM_R_32F, alone_nnz=0):
+        cfg = ServerConfig(cache_bytes, operand_bytes, max_inflight, max_batch, int(plain), btype,
+                           alone_nnz)
         h = C.c_void_p()
         check(lib().dspmm_server_create(C.byref(h), C.byref(cfg)), "server_create")
         self.handle = h.value

@@ -27,7 +27,8 @@ COPY_BYTES = 256 << 20
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c4: the 512-matrix batch in one launch (--n is ignored)")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--bf16", action="store_true")
@@ -43,6 +44,8 @@ def main():
     import dspmm as D
     import workloads as W
 
+    if args.workload == "c4":
+        return batch_c4(args, np, D, W)
     a = {"c1": W.c1_matrix, "c2": W.c2_matrix, "c3": W.c3_matrix}[args.workload]()
     nv = args.n
     B = W.dense_b(2, a.nc, nv)
@@ -72,6 +75,33 @@ def main():
                       "config": cfg.as_dict(), "copy_bytes": COPY_BYTES,
                       "algorithmic_bytes": W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv,
                                                                2 if args.bf16 else 4)}))
+
+
+def batch_c4(args, np, D, W, count=512):
+    """The c4 batch (512 matrices, each with its N, static configuration) in
+    one launch per step, after the same calibration copy."""
+    m, d, n = W.c4_params(count)
+    items = []
+    nnz = alg = 0
+    for i in range(count):
+        a = W.c4_matrix(i, m[i], d[i])
+        nv = int(n[i])
+        mat = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+        items.append((mat, nv, D.DeviceArray.from_numpy(W.dense_b(4000 + i, a.nc, nv)),
+                      D.DeviceArray.zeros((a.nr, nv))))
+        nnz += a.nnz
+        alg += W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv)
+    b = D.Batch(items)
+    src = D.DeviceArray(COPY_BYTES, np.uint8)
+    dst = D.DeviceArray(COPY_BYTES, np.uint8)
+    D.check(D.lib().dspmm_memcpy_d2d(dst.ptr, src.ptr, COPY_BYTES, None), "d2d")
+    D.synchronize()
+    for _ in range(args.steps):
+        b.run()
+    D.synchronize()
+    print(json.dumps({"workload": "c4", "n": 0, "bf16": False, "nnz": nnz,
+                      "config": {"variant": "batched", "matrices": count},
+                      "copy_bytes": COPY_BYTES, "algorithmic_bytes": alg}))
 
 
 if __name__ == "__main__":

@@ -412,8 +412,14 @@ typedef struct dspmm_server_config {
     size_t operand_bytes;   /* HBM arena for server-owned B, C and split carries */
     int max_inflight;       /* batches (slots / streams) in flight, >= 1 */
     int max_batch;          /* requests per batch, >= 1 */
-    int plain;              /* 1: one launch per request, no batching */
+    int plain;              /* 0: continuous batching; 1: one launch per request
+                               with the batch's configuration; 2: one launch per
+                               request with the matrix's own AUTO configuration
+                               (tuned, else selected) */
     dspmm_dtype_t btype;    /* storage of server-generated B (FP32 or BF16) */
+    int64_t alone_nnz;      /* batching mode: requests on matrices with at least
+                               this many nonzeros launch alone with the matrix's
+                               own AUTO configuration (0: 16 Mi nonzeros) */
 } dspmm_server_config;
 
 typedef struct dspmm_request {

@@ -132,3 +132,58 @@ def test_plain_mode_same_bits_as_batched():
         srv.close()
     for k in outs[0]:
         assert np.array_equal(bits(outs[0][k]), bits(outs[1][k])), f"request {k}"
+
+
+def test_plain_own_best_mode_is_correct():
+    """plain = 2 launches each request with its matrix's own AUTO kernel:
+    results within the FP32 gate of the oracle on the device-generated B."""
+    pool = make_pool(4, seed=64)
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16,
+                   plain=2)
+    mids = register(srv, pool)
+    reqs = [srv.request(k, mids[k % len(pool)], pool[k % len(pool)][1], 700 + k)
+            for k in range(2 * len(pool))]
+    srv.replay(reqs, [0.0] * len(reqs))
+    got = results(srv, srv.poll(), pool, {k: k % len(pool) for k in range(len(reqs))})
+    for k in range(len(reqs)):
+        a, nv = pool[k % len(pool)]
+        B = D.seeded_uniform(700 + k, a.nc * nv)
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"plain-own request {k}")
+    srv.close()
+
+
+def test_large_requests_run_alone_with_their_own_kernel():
+    """Requests on matrices above alone_nnz launch one at a time with the
+    matrix's AUTO kernel and a workspace of their own; mixed with batched
+    small requests, every result is within the FP32 gate, and a request
+    gives the same bits each time it runs alone."""
+    pool = make_pool(5, seed=65)                       # the last entry is the split-row Zipf matrix
+    threshold = pool[-1][0].nnz
+    alone = {mi for mi, (a, _) in enumerate(pool) if a.nnz >= threshold}
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16,
+                   alone_nnz=threshold)
+    mids = register(srv, pool)
+    reqs, mi_of = [], {}
+    for k in range(30):
+        mi = k % len(pool)
+        reqs.append(srv.request(k, mids[mi], pool[mi][1], 900 + (k if mi != len(pool) - 1 else 0)))
+        mi_of[k] = mi
+    srv.replay(reqs, [0.0] * len(reqs))
+    comps = srv.poll()
+    assert len(alone) < len(pool)
+    assert all(c["batch_size"] == 1 for c in comps if mi_of[c["id"]] in alone)
+    assert max(c["batch_size"] for c in comps) > 1
+    got = results(srv, comps, pool, mi_of)
+    big_ids = [k for k in mi_of if mi_of[k] == len(pool) - 1]
+    for k in big_ids[1:]:                              # same seed 900: same bits every time
+        assert np.array_equal(bits(got[k]), bits(got[big_ids[0]])), f"request {k}"
+    for k in (0, 1, big_ids[0]):
+        a, nv = pool[mi_of[k]]
+        seed = 900 + (k if mi_of[k] != len(pool) - 1 else 0)
+        B = D.seeded_uniform(seed, a.nc * nv)
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"mixed request {k}")
+    srv.close()
