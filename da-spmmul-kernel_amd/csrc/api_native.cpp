@@ -637,6 +637,34 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
     case DSPMM_VARIANT_CSR_VECTOR:
     case DSPMM_VARIANT_COO_SCALAR:
     case DSPMM_VARIANT_COO_VECTOR: {
+        if (cfg.variant == DSPMM_VARIANT_CSR_SCALAR && layout == DENSE_ROW_MAJOR &&
+            (bf16 || cfg.unroll > 0)) {
+            dspmm_config c = cfg;
+            c.tile_nnz = 0;
+            c.unroll = cfg.unroll > 0 ? cfg.unroll : 4;
+            if (!row_seq_unroll_ok(c.unroll)) return DSPMM_STATUS_NOT_SUPPORTED;
+            complete_nb(c, nv, ldb, ldc, B, C, bf16);
+            const Slab sl = make_slab(nv, c.vec);
+            RowSeqLaunch a;
+            a.rowPtr = A->rowPtr;
+            a.colIdx = A->colIdx;
+            a.values = A->values;
+            a.B = B;
+            a.C = C;
+            a.ldb = ldb;
+            a.ldc = ldc;
+            a.nr = A->nr;
+            a.nv = nv;
+            a.vw = c.vec;
+            a.lanes = std::min(c.lanes, sl.lanes);
+            a.slab_w = a.lanes * c.vec;
+            a.nslabs = div_up(nv, a.slab_w);
+            a.unroll = c.unroll;
+            a.bf16 = bf16;
+            dspmm_status_t s = launch_row_seq(a, st);
+            if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
+            return s;
+        }
         if (bf16) return DSPMM_STATUS_NOT_SUPPORTED;
         const bool rm = layout == DENSE_ROW_MAJOR;
         int which = 0;
@@ -926,12 +954,25 @@ namespace {
 
 // Candidate list of the autotuner: the native nnz-tiled kernel over vector
 // widths (each with the lane count that covers nv in one slab), tile sizes
-// around the default and both unrolls; the SpMV tiles when nv == 1; then
-// the four reference algorithm classes (FP32 B only).
+// around the default and both unrolls; the row-sequential kernel over its
+// unrolls; the row kernels also in column slabs; the SpMV tiles when
+// nv == 1; then the four reference algorithm classes (FP32 B only).
 std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout layout, int nv,
                                               int64_t ldb, int64_t ldc, const void *B,
                                               const float *C, bool bf16) {
     std::vector<dspmm_config> cand;
+    // Lane counts of the nnz-tiled and row-sequential kernels: one slab
+    // covering nv (0 = auto), and half / quarter slabs, which run the columns
+    // in passes (grid.y is the slab, so one pass over all rows ends before
+    // the next begins) and shrink the B footprint a pass keeps in L2 at the
+    // cost of re-reading A.
+    std::vector<int> slab_lanes{0};
+    if (layout == DENSE_ROW_MAJOR && nv > 1) {
+        dspmm_config base{};
+        base.variant = DSPMM_VARIANT_NNZ_TILED;
+        complete_nb(base, nv, ldb, ldc, B, C, bf16);
+        for (int l = base.lanes / 2; l >= 1 && l * 4 >= base.lanes; l /= 2) slab_lanes.push_back(l);
+    }
     if (layout == DENSE_ROW_MAJOR && nv > 1) {
         // column-cached tiles, only where neighbouring rows share columns
         double reuse;
@@ -978,7 +1019,7 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
             complete_nb(c, nv, ldb, ldc, B, C, bf16);
             if (c.vec != vw) continue;
             const int s0 = default_tile(c.lanes);
-            for (int S : {s0 / 4, s0 / 2, s0, s0 * 2, s0 * 4}) {
+            for (int S : {s0 / 8, s0 / 4, s0 / 2, s0, s0 * 2, s0 * 4}) {
                 if (S < 8 || S > 2048) continue;
                 if (nnz_tiled_lds_bytes(c.lanes, S) > kMaxTileLds) continue;
                 for (int u : {4, 8}) {
@@ -988,7 +1029,29 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                     cand.push_back(x);
                 }
             }
+            if (vw == base.vec && slab_lanes.size() > 1) {   // half slabs, default tiles
+                dspmm_config x = c;
+                x.lanes = slab_lanes[1];
+                x.tile_nnz = default_tile(x.lanes);
+                if (nnz_tiled_lds_bytes(x.lanes, x.tile_nnz) <= kMaxTileLds)
+                    for (int u : {4, 8}) {
+                        x.unroll = u;
+                        cand.push_back(x);
+                    }
+            }
         }
+    }
+    if (layout == DENSE_ROW_MAJOR) {
+        // row-sequential (LDS-staged indices), over the gathers in flight
+        for (int u : {2, 4, 8, 16})
+            for (int lanes : slab_lanes) {
+                if (lanes && (u == 2 || u == 16)) continue;
+                dspmm_config c{};
+                c.variant = DSPMM_VARIANT_CSR_SCALAR;
+                c.unroll = u;
+                c.lanes = lanes;
+                cand.push_back(c);
+            }
     }
     if (nv == 1) {
         for (int T : {512, 1024, 2048, 4096}) {

@@ -76,6 +76,22 @@ struct RowBlockLaunch {
 };
 dspmm_status_t launch_row_blocked(const RowBlockLaunch &a, hipStream_t st);
 
+// Arguments of one row-sequential launch (a lane group per row, indices
+// staged per wavefront in LDS, `unroll` gathers in flight).
+struct RowSeqLaunch {
+    const int *rowPtr = nullptr;
+    const int *colIdx = nullptr;
+    const float *values = nullptr;
+    const void *B = nullptr;
+    float *C = nullptr;
+    int64_t ldb = 0, ldc = 0;
+    int nr = 0, nv = 0;
+    int vw = 1, lanes = 1, slab_w = 1, nslabs = 1, unroll = 4;
+    bool bf16 = false;
+};
+bool row_seq_unroll_ok(int u);
+dspmm_status_t launch_row_seq(const RowSeqLaunch &a, hipStream_t st);
+
 // Arguments of one single-matrix nnz-tiled launch (plus its split combine).
 struct NbLaunch {
     ItemDesc item{};

@@ -585,16 +585,23 @@ __global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
 // with the host loop -- while the B rows a block's rows share are gathered
 // once instead of once per row (a 27-point stencil: 9 (R + 2) gathers for
 // 27 R nonzeros).
+//
+// A wavefront owns G = 64 / L consecutive row blocks.  Their list entries
+// and values are contiguous, so the wavefront stages them into its own LDS
+// region in windows of kRbWin entries with coalesced 16-byte loads; the
+// groups then read their (col, mask, values) from LDS as broadcasts.  The
+// only per-entry vector-memory instruction left is the B-row gather, which
+// is what the L1 address/data path is busy with.
+constexpr int kRbWin = 128;
+
 template <int R>
-__device__ __forceinline__ void load_row_vals(float (&v)[R], const float *p) {
-    if constexpr (R == 8) {
-        const float4 a = reinterpret_cast<const float4 *>(p)[0];
-        const float4 b = reinterpret_cast<const float4 *>(p)[1];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else if constexpr (R == 4) {
-        const float4 a = *reinterpret_cast<const float4 *>(p);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+__device__ __forceinline__ void lds_row_vals(float (&v)[R], const float *p) {
+    if constexpr (R >= 4) {
+#pragma unroll
+        for (int q = 0; q < R / 4; q++) {
+            const float4 a = reinterpret_cast<const float4 *>(p)[q];
+            v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+        }
     } else {
         const float2 a = *reinterpret_cast<const float2 *>(p);
         v[0] = a.x; v[1] = a.y;
@@ -603,48 +610,77 @@ __device__ __forceinline__ void load_row_vals(float (&v)[R], const float *p) {
 
 template <typename TB, int VW, int L, int R, int UNR>
 __global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
-    constexpr int G = kBlock / L;
-    const int t = threadIdx.x;
-    const int blk = xcd_remap(blockIdx.x, gridDim.x) * G + t / L;
-    const int col0 = blockIdx.y * a.slab_w + (t % L) * VW;
-    if (blk >= a.nblocks || col0 >= a.nv) return;
-    const int r0 = blk * R;
-    const int u0 = a.boff[blk], u1 = a.boff[blk + 1];
-    const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B);
+    constexpr int G = kWave / L;                 // row blocks per wavefront
+    constexpr int WPB = kBlock / kWave;          // wavefronts per workgroup
+    static_assert(R == 2 || R == 4 || R == 8, "R");
+    __shared__ int2 s_list[WPB][kRbWin];
+    __shared__ __align__(16) float s_vals[WPB][kRbWin * R];
+    const int t = threadIdx.x, wv = t / kWave, lane = t % kWave, g = lane / L;
+    const int blk_w = (xcd_remap(blockIdx.x, gridDim.x) * WPB + wv) * G;
+    if (blk_w >= a.nblocks) return;              // the whole wavefront leaves
+    const int nb = min(G, a.nblocks - blk_w);
+    const int blk = blk_w + g;
+    const int col0 = blockIdx.y * a.slab_w + (lane % L) * VW;
+    const bool active = g < nb && col0 < a.nv;
+    const int e0 = a.boff[blk_w], e1 = a.boff[blk_w + nb];
+    int u0 = 0, u1 = 0;
+    if (g < nb) {
+        u0 = a.boff[blk];
+        u1 = a.boff[blk + 1];
+    }
+    const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B) + col0;
+    int2 *sl = s_list[wv];
+    float *sv = s_vals[wv];
     float acc[R][VW];
 #pragma unroll
     for (int i = 0; i < R; i++)
 #pragma unroll
         for (int k = 0; k < VW; k++) acc[i][k] = 0.f;
-    for (int u = u0; u < u1; u += UNR) {
-        int2 cm[UNR];
-        float bv[UNR][VW], v[UNR][R];
-#pragma unroll
-        for (int x = 0; x < UNR; x++) {
-            cm[x] = make_int2(0, 0);
-            if (u + x < u1) {
-                cm[x] = a.list[u + x];
-                load_row_vals<R>(v[x], a.vals + (int64_t)(u + x) * R);
-            }
+    for (int w = e0; w < e1; w += kRbWin) {
+        const int cnt = min(kRbWin, e1 - w);
+        wave_lds_sync();
+        for (int i = lane; i < cnt; i += kWave) sl[i] = a.list[w + i];
+        if constexpr (R >= 4) {
+            const float4 *src = reinterpret_cast<const float4 *>(a.vals + (int64_t)w * R);
+            float4 *dst = reinterpret_cast<float4 *>(sv);
+            for (int i = lane; i < cnt * (R / 4); i += kWave) dst[i] = src[i];
+        } else {
+            const float2 *src = reinterpret_cast<const float2 *>(a.vals + (int64_t)w * R);
+            float2 *dst = reinterpret_cast<float2 *>(sv);
+            for (int i = lane; i < cnt; i += kWave) dst[i] = src[i];
         }
+        wave_lds_sync();
+        if (!active) continue;
+        const int p0 = max(u0, w) - w, p1 = min(u1, w + cnt) - w;
+        for (int u = p0; u < p1; u += UNR) {
+            int2 cm[UNR];
+            float bv[UNR][VW];
 #pragma unroll
-        for (int x = 0; x < UNR; x++)
-            if (u + x < u1) load_b<TB, VW>(bv[x], B + (int64_t)cm[x].x * a.ldb + col0);
+            for (int x = 0; x < UNR; x++) {
+                cm[x] = sl[min(u + x, p1 - 1)];          // padded: no branch around loads
+                if (u + x >= p1) cm[x].y = 0;
+            }
 #pragma unroll
-        for (int x = 0; x < UNR; x++) {
-            if (u + x < u1) {
+            for (int x = 0; x < UNR; x++) load_b<TB, VW>(bv[x], B + (int64_t)cm[x].x * a.ldb);
+#pragma unroll
+            for (int x = 0; x < UNR; x++) {
+                float v[R];
+                lds_row_vals<R>(v, sv + min(u + x, p1 - 1) * R);
 #pragma unroll
                 for (int i = 0; i < R; i++) {
-                    const bool has = (cm[x].y >> i) & 1;
+                    // a branch, not a select: groups of one wavefront usually
+                    // share the mask pattern, so rows without the column cost
+                    // no vector instructions
+                    if ((cm[x].y >> i) & 1) {
 #pragma unroll
-                    for (int k = 0; k < VW; k++) {
-                        const float sum = acc[i][k] + v[x][i] * bv[x][k];
-                        acc[i][k] = has ? sum : acc[i][k];
+                        for (int k = 0; k < VW; k++) acc[i][k] = acc[i][k] + v[i] * bv[x][k];
                     }
                 }
             }
         }
     }
+    if (!active) return;
+    const int r0 = blk * R;
 #pragma unroll
     for (int i = 0; i < R; i++)
         if (r0 + i < a.nr) store_vec_nt<VW>(a.C + (int64_t)(r0 + i) * a.ldc + col0, acc[i]);
@@ -873,7 +909,7 @@ dspmm_status_t launch_nnz_cached(const CachedLaunch &a, hipStream_t st) {
 namespace {
 template <typename TB, int VW, int L, int R>
 dspmm_status_t launch_rb_r(const RowBlockLaunch &a, hipStream_t st) {
-    constexpr int G = kBlock / L;
+    constexpr int G = kBlock / L;              // row blocks per workgroup
     const dim3 grid(div_up(a.nblocks, G), a.nslabs);
     if (a.unroll <= 4) hipLaunchKernelGGL((k_row_blocked<TB, VW, L, R, 4>), grid, dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((k_row_blocked<TB, VW, L, R, 8>), grid, dim3(kBlock), 0, st, a);
@@ -918,6 +954,116 @@ dspmm_status_t launch_row_blocked(const RowBlockLaunch &a, hipStream_t st) {
     case 4: return launch_rb_v<float, 4>(a, st);
     case 2: return launch_rb_v<float, 2>(a, st);
     case 1: return launch_rb_v<float, 1>(a, st);
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+}
+
+// ===========================================================================
+// Row-sequential SpMM (CSR_SCALAR with an unroll, and its BF16-B form).  A
+// group of 2^lg lanes owns one row, lane j of the group VW consecutive
+// columns of the slab; a wavefront owns G = 64 >> lg consecutive rows.  The
+// wavefront stages its rows' (col, val) into its own LDS region in windows
+// of kRsWin nonzeros with coalesced loads; each group then reads its row's
+// pairs from LDS as broadcasts, issues UNR gathers of B and adds the
+// products in storage order, so results equal the host loop's row sums.
+// The last chunk of a window is padded with its last entry (loads without
+// a branch keep the gathers back to back) and the padded products are
+// skipped.  No workgroup barriers; every row is overwritten (empty rows
+// -> 0).
+// ===========================================================================
+constexpr int kRsWin = 512;
+
+template <typename TB, int VW, int UNR>
+__global__ __launch_bounds__(kBlock) void k_row_seq(RowSeqLaunch g, int lg) {
+    constexpr int WPB = kBlock / kWave;
+    __shared__ int s_col[WPB][kRsWin];
+    __shared__ float s_val[WPB][kRsWin];
+    const int t = threadIdx.x, wv = t / kWave, lane = t % kWave;
+    const int G = kWave >> lg;
+    const int64_t rw = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * WPB + wv) * G;
+    if (rw >= g.nr) return;                      // the whole wavefront leaves
+    const int nrow = (int)min((int64_t)G, (int64_t)g.nr - rw);
+    const int gi = lane >> lg;
+    const int64_t r = rw + gi;
+    const int col0 = blockIdx.y * g.slab_w + (lane & ((1 << lg) - 1)) * VW;
+    const bool active = gi < nrow && col0 < g.nv;
+    const int e0 = g.rowPtr[rw], e1 = g.rowPtr[rw + nrow];
+    int start = 0, end = 0;
+    if (gi < nrow) {
+        start = g.rowPtr[r];
+        end = g.rowPtr[r + 1];
+    }
+    const TB *__restrict__ B = reinterpret_cast<const TB *>(g.B) + col0;
+    int *sc = s_col[wv];
+    float *sv = s_val[wv];
+    float acc[VW];
+#pragma unroll
+    for (int k = 0; k < VW; k++) acc[k] = 0.f;
+    for (int w = e0; w < e1; w += kRsWin) {
+        const int cnt = min(kRsWin, e1 - w);
+        wave_lds_sync();
+        for (int i = lane; i < cnt; i += kWave) {
+            sc[i] = ld_stream(g.colIdx + w + i);
+            sv[i] = ld_stream(g.values + w + i);
+        }
+        wave_lds_sync();
+        if (!active) continue;
+        const int p0 = max(start, w) - w, p1 = min(end, w + cnt) - w;
+        for (int p = p0; p < p1; p += UNR) {
+            float bv[UNR][VW];
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                load_b<TB, VW>(bv[u], B + (int64_t)sc[min(p + u, p1 - 1)] * g.ldb);
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                if (p + u < p1) {
+                    const float a = sv[p + u];
+#pragma unroll
+                    for (int k = 0; k < VW; k++) acc[k] = acc[k] + a * bv[u][k];
+                }
+            }
+        }
+    }
+    if (active) store_vec<VW>(g.C + r * g.ldc + col0, acc);
+}
+
+namespace {
+template <typename TB, int VW>
+dspmm_status_t launch_row_seq_v(const RowSeqLaunch &a, hipStream_t st) {
+    int lg = 0;
+    while ((1 << lg) < a.lanes) lg++;
+    const dim3 grid(div_up(a.nr, kBlock >> lg), a.nslabs);
+    switch (a.unroll) {
+    case 2: hipLaunchKernelGGL((k_row_seq<TB, VW, 2>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 4: hipLaunchKernelGGL((k_row_seq<TB, VW, 4>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 8: hipLaunchKernelGGL((k_row_seq<TB, VW, 8>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 16: hipLaunchKernelGGL((k_row_seq<TB, VW, 16>), grid, dim3(kBlock), 0, st, a, lg); break;
+    default: return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+    return launch_status();
+}
+}  // namespace
+
+bool row_seq_unroll_ok(int u) { return u == 2 || u == 4 || u == 8 || u == 16; }
+
+dspmm_status_t launch_row_seq(const RowSeqLaunch &a, hipStream_t st) {
+    if (a.nr == 0 || a.nv == 0) return DSPMM_STATUS_SUCCESS;
+    if (!row_seq_unroll_ok(a.unroll) || a.lanes < 1 || a.lanes > 64 || (a.lanes & (a.lanes - 1)) ||
+        a.slab_w != a.lanes * a.vw || (int64_t)a.nslabs * a.slab_w < a.nv)
+        return DSPMM_STATUS_NOT_SUPPORTED;
+    if (a.bf16) {
+        switch (a.vw) {
+        case 8: return launch_row_seq_v<uint16_t, 8>(a, st);
+        case 4: return launch_row_seq_v<uint16_t, 4>(a, st);
+        case 2: return launch_row_seq_v<uint16_t, 2>(a, st);
+        case 1: return launch_row_seq_v<uint16_t, 1>(a, st);
+        default: return DSPMM_STATUS_NOT_SUPPORTED;
+        }
+    }
+    switch (a.vw) {
+    case 4: return launch_row_seq_v<float, 4>(a, st);
+    case 2: return launch_row_seq_v<float, 2>(a, st);
+    case 1: return launch_row_seq_v<float, 1>(a, st);
     default: return DSPMM_STATUS_NOT_SUPPORTED;
     }
 }

@@ -239,7 +239,13 @@ typedef enum dspmm_variant {
     DSPMM_VARIANT_AUTO = 0,     /* cached autotune result, else heuristic */
     DSPMM_VARIANT_NNZ_TILED = 1,/* nnz-balanced row-aligned tiles, sequential
                                    per-row sums, deterministic, overwrite  */
-    DSPMM_VARIANT_CSR_SCALAR = 2,
+    DSPMM_VARIANT_CSR_SCALAR = 2,/* a lane group per row, sequential sum (the
+                                    host loop's order); row-major with unroll
+                                    2/4/8/16 or BF16 B: each wavefront stages
+                                    its rows' (col, val) in LDS and keeps
+                                    `unroll` gathers in flight per group;
+                                    unroll 0 with FP32 B: the reference-class
+                                    kernel (workgroup-staged, 4 in flight) */
     DSPMM_VARIANT_CSR_VECTOR = 3,
     DSPMM_VARIANT_COO_SCALAR = 4,
     DSPMM_VARIANT_COO_VECTOR = 5,

@@ -336,3 +336,50 @@ def test_row_blocked_rejects_repeated_columns():
     D.synchronize()
     assert np.array_equal(d_C.numpy(), np.array([[3] * 4, [1] * 4], np.float32))
     m.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20", "int_valued"])
+@pytest.mark.parametrize("unroll", [2, 4, 8, 16])
+def test_row_sequential(name, unroll):
+    """CSR_SCALAR with an unroll (direct index loads, padded last chunk):
+    each row is summed in nonzero order, so every output equals the host
+    loop bit for bit -- FP32 and BF16-stored B, any nv, and several column
+    slabs when the lane count is capped."""
+    c, m = matrix(name)
+    cfg = D.make_config(D.VARIANT_CSR_SCALAR, unroll=unroll)
+    for nv in (1, 3, 8, 64, 128):
+        if nv > c.maxnv:
+            continue
+        h32, c64, ab = c.ref(RM, nv)
+        what = f"{name} row-seq unroll={unroll} nv={nv}"
+        note(what)
+        assert_bitwise(run_native(c, m, nv, config=cfg), h32, what)
+    nv = min(64, c.maxnv)
+    h32, c64, ab = c.ref(RM, nv)
+    narrow = D.make_config(D.VARIANT_CSR_SCALAR, vec=2, lanes=4, unroll=unroll)   # 8 slabs
+    assert_bitwise(run_native(c, m, nv, config=narrow), h32, f"{name} row-seq slabs")
+    d16 = D.to_bf16_device(c.d_B)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(c.dense(nv))).reshape(-1)
+    for bf in (cfg, D.make_config(D.VARIANT_CSR_SCALAR)):
+        out = run_native(c, m, nv, config=bf, btype=D.DSPMM_R_16BF, B=d16)
+        assert_bitwise(out, h32r, f"{name} row-seq unroll={bf.unroll} bf16")
+
+
+def test_row_sequential_leading_dimensions_and_bad_unroll():
+    c, m = matrix("zipf_mid")
+    nv, ldb, ldc = 24, 40, 36
+    B = W.gen_dense(78, c.nc * ldb).reshape(c.nc, ldb)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.from_numpy(np.full((c.nr, ldc), -3.0, np.float32))
+    m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc, config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=8))
+    D.synchronize()
+    out = d_C.numpy()
+    assert np.all(out[:, nv:] == -3.0), "columns beyond nv must be left alone"
+    h32 = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                        np.ascontiguousarray(B[:, :nv]))
+    assert_bitwise(np.ascontiguousarray(out[:, :nv]).reshape(-1), h32.reshape(-1), "row-seq ld")
+    with pytest.raises(D.DspmmError) as e:
+        m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc,
+               config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=5))
+    assert e.value.status == D.STATUS_NOT_SUPPORTED
