@@ -2,7 +2,9 @@
 """Turns rocprofv3 output of profile_workload.py into a short summary.
 
 Inputs: a kernel-trace results database (or kernel_stats CSV) and the two
-counter-collection CSVs (FETCH_SIZE, WRITE_SIZE).  Output: a JSON file with,
+counter-collection CSVs (FETCH_SIZE, WRITE_SIZE); or, in the counter mode
+(--pmc), any counter-collection CSVs, summarised per kernel with derived
+busy fractions, hit rates and L1-miss latency.  Output: a JSON file with,
 per kernel, the average duration and the average FETCH/WRITE KiB per
 dispatch, plus the HBM traffic per dispatch scaled by the copy calibration
 (the device-to-device copy of known size in the same run), and a Markdown
@@ -89,9 +91,71 @@ def trace_only(args):
     print(json.dumps({"out": args.out}))
 
 
+# Derived figures of the counter mode, from the raw counters they need.
+def derived(v, cus, xcds):
+    out = {}
+    cyc = v.get("GRBM_GUI_ACTIVE")
+    per_cu = cyc / xcds if cyc else None          # the counter sums the XCDs' clocks
+    if per_cu:
+        for name, key in (("TA busy", "TA_TA_BUSY_sum"), ("TD busy", "TD_TD_BUSY_sum"),
+                          ("TD stalled on L1", "TD_TC_STALL_sum")):
+            if key in v:
+                out[name] = v[key] / cus / per_cu
+    if "TCP_TCC_READ_REQ_sum" in v and v.get("TCP_TOTAL_CACHE_ACCESSES_sum"):
+        out["L1 hit rate"] = 1.0 - v["TCP_TCC_READ_REQ_sum"] / v["TCP_TOTAL_CACHE_ACCESSES_sum"]
+    if "TCC_HIT_sum" in v and "TCC_MISS_sum" in v:
+        out["L2 hit rate"] = v["TCC_HIT_sum"] / max(1.0, v["TCC_HIT_sum"] + v["TCC_MISS_sum"])
+    if v.get("TCP_TCC_READ_REQ_sum") and "TCP_TCC_READ_REQ_LATENCY_sum" in v:
+        out["L1-miss latency (cycles)"] = (v["TCP_TCC_READ_REQ_LATENCY_sum"] /
+                                           v["TCP_TCC_READ_REQ_sum"])
+    if per_cu and "TCP_TCC_READ_REQ_sum" in v and "L1-miss latency (cycles)" in out:
+        # Little's law: L1 misses in flight per CU
+        out["L1 misses in flight per CU"] = (v["TCP_TCC_READ_REQ_sum"] / cus / per_cu *
+                                             out["L1-miss latency (cycles)"])
+    if v.get("SQ_INSTS_VMEM_RD") and v.get("SQ_WAVES"):
+        out["vector-memory reads per wave"] = v["SQ_INSTS_VMEM_RD"] / v["SQ_WAVES"]
+    return out
+
+
+def pmc_mode(args):
+    """Per kernel (fills and copies aside): the average of every counter per
+    dispatch over all given counter-collection CSVs, and derived figures."""
+    tables = []
+    for spec in args.pmc:
+        tag, pattern = spec.split("=", 1)
+        vals = defaultdict(lambda: defaultdict(list))
+        for path in sorted(glob.glob(pattern)):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    k = short(r["Kernel_Name"])
+                    if any(x in k.lower() for x in ("fill", "copy", "to_bf16")):
+                        continue
+                    vals[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for k, cs in vals.items():
+            avg = {c: sum(x) / len(x) for c, x in cs.items()}
+            tables.append((tag, k, avg, derived(avg, args.cus, args.xcds)))
+    with open(args.out, "w") as f:
+        f.write(f"# rocprofv3 counters: {args.title}\n\n")
+        f.write(f"Command: `{args.command}`\n\n")
+        f.write(f"Per-dispatch averages; busy fractions are per CU over the kernel's clock "
+                f"({args.cus} CUs, {args.xcds} XCDs).\n\n")
+        for tag, k, avg, d in tables:
+            f.write(f"## {tag}: `{k}`\n\n| figure | value |\n|---|---|\n")
+            for name, x in d.items():
+                f.write(f"| {name} | {x:.3g} |\n")
+            for name, x in sorted(avg.items()):
+                f.write(f"| {name} | {x:.4g} |\n")
+            f.write("\n")
+    print(json.dumps({"out": args.out, "kernels": [t[1] for t in tables]}))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--trace", required=True, help="kernel-trace .db or kernel_stats.csv")
+    ap.add_argument("--trace", help="kernel-trace .db or kernel_stats.csv")
+    ap.add_argument("--pmc", nargs="*", default=[],
+                    help="counter mode: TAG=GLOB of counter_collection CSVs, one per config")
+    ap.add_argument("--cus", type=int, default=256)
+    ap.add_argument("--xcds", type=int, default=8)
     ap.add_argument("--fetch", help="counter_collection.csv with FETCH_SIZE")
     ap.add_argument("--write", help="counter_collection.csv with WRITE_SIZE")
     ap.add_argument("--meta", help="JSON line printed by profile_workload.py")
@@ -103,6 +167,10 @@ def main():
                     "last --last dispatches are averaged (the timed steps)")
     ap.add_argument("--last", type=int, default=0)
     args = ap.parse_args()
+    if args.pmc:
+        return pmc_mode(args)
+    if not args.trace:
+        raise SystemExit("--trace is required outside the counter mode")
     if not args.fetch:
         return trace_only(args)
 
