@@ -1,0 +1,152 @@
+"""GPU parity at the full sizes of the measured workloads, through properties
+that do not need the oracle to run the whole product:
+
+* kernels that each sum every row in nonzero order (row-exact) must agree bit
+  for bit on every output of the full matrix;
+* a sample of rows equals the host loop (the oracle run on the sampled rows'
+  sub-matrix) bit for bit, and the FP64 oracle within the FP32 gate;
+* the batched launch over all 512 matrices of the batch workload equals one
+  plain launch per matrix bit for bit;
+* repeat calls give the same bits.
+"""
+import numpy as np
+import pytest
+
+import dspmm as D
+import oracle as O
+import workloads as W
+from gpu_common import assert_bitwise, assert_fp32, bits, note
+
+pytestmark = pytest.mark.gpu
+RM = D.DENSE_ROW_MAJOR
+
+
+def sample(a, count, seed):
+    """Sub-matrix of `count` random rows: (rows, rowPtr, colIdx, values)."""
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(a.nr, size=min(count, a.nr), replace=False))
+    lens = (a.rowPtr[rows + 1] - a.rowPtr[rows]).astype(np.int64)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.repeat(a.rowPtr[rows].astype(np.int64) - rp[:-1], lens) + np.arange(rp[-1])
+    return rows, rp, a.colIdx[idx], a.values[idx]
+
+
+def check_sampled_rows(a, sub, B, nv, out, what, exact=True):
+    rows, rp, ci, v = sub
+    h32 = O.spmm_host32(RM, rows.size, a.nc, nv, rp, ci, v, B).reshape(-1)
+    got = np.ascontiguousarray(out.reshape(a.nr, nv)[rows]).reshape(-1)
+    if exact:
+        assert_bitwise(got, h32, what + " (sampled rows vs host loop)")
+    c64, ab = O.spmm_fp64(RM, rows.size, a.nc, nv, rp, ci, v, B)
+    rl = np.repeat(np.diff(rp).astype(np.float64), nv)
+    assert_fp32(got, c64.reshape(-1), ab.reshape(-1), rl, what + " (sampled rows vs FP64)", h32)
+
+
+def fill_nan(d):
+    host = np.full(d.nbytes // 4, np.nan, np.float32)
+    D.check(D.lib().dspmm_memcpy_h2d(d.ptr, host.ctypes.data, host.nbytes), "h2d")
+
+
+def run(m, d_B, d_C, nv, cfg, btype=D.DSPMM_R_32F):
+    fill_nan(d_C)
+    m.spmm(d_B, d_C, nv, btype=btype, config=cfg)
+    D.synchronize()
+    return d_C.numpy().reshape(-1).copy()
+
+
+def test_stencil_100_cubed_row_exact_kernels_agree():
+    """27-point stencil on a 100^3 grid (M = 1e6, 26.5 M nonzeros), N = 64:
+    four row-exact kernels give the same 64 M outputs; sampled rows equal the
+    host loop; the same with BF16-stored B."""
+    a = W.c3_matrix(100)
+    nv = 64
+    B = W.dense_b(3, a.nc, nv)
+    m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.zeros((a.nr, nv))
+    cfgs = {"reference-class SR_RB": D.make_config(D.VARIANT_CSR_SCALAR),
+            "row-sequential": D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
+            "nnz-tiled S=64": D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=64),
+            "row-blocked R=2": D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4)}
+    first = None
+    for name, cfg in cfgs.items():
+        note(f"c3 full {name}")
+        out = run(m, d_B, d_C, nv, cfg)
+        if first is None:
+            first = out
+            check_sampled_rows(a, sample(a, 3000, 31), B, nv, out, f"c3 {name}")
+        else:
+            assert np.array_equal(bits(out), bits(first)), f"c3 {name} differs from SR_RB"
+    again = run(m, d_B, d_C, nv, cfgs["row-sequential"])
+    assert np.array_equal(bits(again), bits(first)), "repeat call differs"
+    d16 = D.to_bf16_device(d_B)
+    outs = [run(m, d16, d_C, nv, cfg, D.DSPMM_R_16BF) for cfg in
+            (D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
+             D.make_config(D.VARIANT_NNZ_TILED, vec=8, lanes=8, tile_nnz=32, unroll=4))]
+    assert np.array_equal(bits(outs[0]), bits(outs[1])), "c3 BF16: row kernels differ"
+    rows, rp, ci, v = sample(a, 3000, 32)
+    h32 = O.spmm_host32(RM, rows.size, a.nc, nv, rp, ci, v, O.round_bf16(B)).reshape(-1)
+    got = np.ascontiguousarray(outs[0].reshape(a.nr, nv)[rows]).reshape(-1)
+    assert_bitwise(got, h32, "c3 BF16 sampled rows vs host loop on the rounded B")
+    m.close()
+
+
+def test_large_power_law_row_exact_kernels_agree():
+    """The served large matrix (M = K = 2e6, power-law rows, ~40 M nonzeros),
+    N = 64: the reference-class and row-sequential kernels agree on every
+    output and equal the host loop on sampled rows; the AUTO kernel (which
+    splits long rows) is within the FP32 gate on the same rows."""
+    a = W.c5_large_matrix()
+    nv = 64
+    note(f"c5 large: M={a.nr} nnz={a.nnz}")
+    B = W.dense_b(5, a.nc, nv)
+    m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.zeros((a.nr, nv))
+    ref = run(m, d_B, d_C, nv, D.make_config(D.VARIANT_CSR_SCALAR))
+    seq = run(m, d_B, d_C, nv, D.make_config(D.VARIANT_CSR_SCALAR, unroll=8))
+    assert np.array_equal(bits(ref), bits(seq)), "c5 large: row kernels differ"
+    sub = sample(a, 2000, 51)
+    check_sampled_rows(a, sub, B, nv, ref, "c5 large SR_RB")
+    auto = run(m, d_B, d_C, nv, None)
+    check_sampled_rows(a, sub, B, nv, auto, "c5 large AUTO", exact=False)
+    m.close()
+
+
+def test_batch_of_512_equals_plain_launches():
+    """All 512 matrices of the batch workload (M = K in [256, 16384], N in
+    {16 .. 256}): one batched launch equals one plain launch per matrix with
+    the same configuration, bit for bit; a sample of matrices is checked
+    against the FP64 oracle."""
+    count = 512
+    ms, ds, ns = W.c4_params(count)
+    items, keep = [], []
+    for i in range(count):
+        a = W.c4_matrix(i, ms[i], ds[i])
+        nv = int(ns[i])
+        mat = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+        B = W.dense_b(4000 + i, a.nc, nv)
+        d_B = D.DeviceArray.from_numpy(B)
+        d_C = D.DeviceArray.zeros((a.nr, nv))
+        items.append((mat, nv, d_B, d_C))
+        keep.append((a, B))
+    b = D.Batch(items)
+    note(f"c4 batch: {b.describe()}")
+    b.run()
+    D.synchronize()
+    batched = [it[3].numpy().reshape(-1).copy() for it in items]
+    for it in items:
+        fill_nan(it[3])
+    b.run_plain()
+    D.synchronize()
+    for i, it in enumerate(items):
+        assert np.array_equal(bits(batched[i]), bits(it[3].numpy().reshape(-1))), f"matrix {i}"
+    for i in np.random.default_rng(7).choice(count, 6, replace=False):
+        a, B = keep[i]
+        nv = items[i][1]
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        assert_fp32(batched[i], c64.reshape(-1), ab.reshape(-1), rl, f"c4 matrix {i}")
+    b.close()
+    for it in items:
+        it[0].close()
