@@ -365,6 +365,18 @@ def main():
                 sweep[D.CSV_NAMES[(alg, layout)]] = fl / (t_ms * 1e-3) / 1e9
         extra[f"reference_algorithms_N{nv}_GFLOP/s"] = sweep
         log(f"reference algorithms N={nv}: {sweep}")
+        # Column-major operands: the autotuned native kernel (row-major kernels
+        # on transposed copies, or a reference class) against the best
+        # reference-class column-major kernel above.
+        cm = D.DENSE_COL_MAJOR
+        best_cm, _ = m.autotune(d_B, d_C, nv, layout=cm, reps=3)
+        t_cm, _, _ = m.benchmark(d_B, d_C, nv, layout=cm, config=best_cm, warmup=args.warmup,
+                                 reps=min(args.steps, 50), per_call=False)
+        ref_cm = max(v for k, v in sweep.items() if k.endswith("_CM"))
+        extra[f"column_major_N{nv}"] = {"config": best_cm.as_dict(), "ms": t_cm,
+                                        "GFLOP/s": fl / (t_cm * 1e-3) / 1e9,
+                                        "best_reference_class_GFLOP/s": ref_cm}
+        log(f"column-major N={nv}: {extra[f'column_major_N{nv}']}")
         extra["rocsparse_c2"] = bench_rocsparse(D, a, d_rp, d_ri, d_ci, d_v, d_B, d_C,
                                                 sorted({1, 32, nv}))
         log(f"rocsparse: {extra['rocsparse_c2']}")

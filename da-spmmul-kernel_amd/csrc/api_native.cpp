@@ -165,6 +165,7 @@ void free_matrix(dspmm_matrix_s *A) {
         hipFree(kv.second.d_vals);
     }
     hipFree(A->carry);
+    hipFree(A->tbuf);
     delete A;
 }
 
@@ -400,19 +401,46 @@ constexpr int kSpmvTile = 1024;
 constexpr int kCachedUmaxDefault = 128;
 constexpr double kCachedMinReuse = 1.5;   // sampled nonzeros per distinct column
 constexpr double kRowBlockMinReuse = 1.2;
+// Column-major work (nnz x nv) from which the static default transposes the
+// operands for the row-major nnz-tiled kernel instead of running the
+// reference-class column-major kernel: below it, the two transposes cost
+// more than they save.
+constexpr int64_t kTransposedMinWork = 1 << 20;
 
 dspmm_config heuristic(const dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt) {
-    (void)A;
     dspmm_config c{};
     if (nv == 1) {
         c.variant = DSPMM_VARIANT_SPMV_TILED;
         c.tile_nnz = kSpmvTile;
-    } else if (layout == DENSE_ROW_MAJOR || bt == DSPMM_R_16BF) {
-        c.variant = DSPMM_VARIANT_NNZ_TILED;
+    } else if (layout == DENSE_ROW_MAJOR || bt == DSPMM_R_16BF ||
+               (int64_t)A->nnz * nv >= kTransposedMinWork) {
+        c.variant = DSPMM_VARIANT_NNZ_TILED;        // column-major: through transposed copies
     } else {
         c.variant = DSPMM_VARIANT_CSR_SCALAR;
     }
     return c;
+}
+
+// Row-major-only native kernels: on column-major operands they run on
+// transposed copies of B and C.
+bool rm_native(const dspmm_config &c, bool bf16) {
+    switch (c.variant) {
+    case DSPMM_VARIANT_NNZ_TILED:
+    case DSPMM_VARIANT_ROW_BLOCKED:
+    case DSPMM_VARIANT_NNZ_CACHED: return true;
+    case DSPMM_VARIANT_CSR_SCALAR: return c.unroll > 0 || bf16;
+    default: return false;
+    }
+}
+
+int64_t transposed_ld(int nv) { return ((int64_t)nv + 7) / 8 * 8; }
+
+// Scratch of the transposed path: B^T (nc x ld) then C (nr x ld), each
+// rounded up to 256 bytes.
+size_t transposed_bytes(const dspmm_matrix_s *A, int nv, bool bf16) {
+    const size_t ld = (size_t)transposed_ld(nv);
+    const size_t b = (size_t)A->nc * ld * (bf16 ? 2 : 4), c = (size_t)A->nr * ld * 4;
+    return (b + 255) / 256 * 256 + (c + 255) / 256 * 256;
 }
 
 __global__ void k_zero_2d(float *__restrict__ p, int rows, int cols, int64_t ld) {
@@ -439,6 +467,38 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
                       dspmm_dtype_t bt, float *C, int64_t ldc, const dspmm_config &cfg,
                       void *ws, size_t ws_bytes, hipStream_t st, hipEvent_t mid = nullptr) {
     const bool bf16 = bt == DSPMM_R_16BF;
+    if (layout == DENSE_COL_MAJOR && rm_native(cfg, bf16)) {
+        if (nv == 1)      // x and y are contiguous: the row-major call with unit strides
+            return run_ex(A, DENSE_ROW_MAJOR, 1, B, 1, bt, C, 1, cfg, ws, ws_bytes, st, mid);
+        const size_t tb = transposed_bytes(A, nv, bf16);
+        char *base = nullptr;
+        if (ws) {
+            if (ws_bytes < tb) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
+            if (!aligned(ws, 256)) return DSPMM_STATUS_INVALID_VALUE;
+            base = reinterpret_cast<char *>(ws);
+        } else {
+            if (A->tbuf_bytes < tb) {
+                hipFree(A->tbuf);
+                A->tbuf = nullptr;
+                A->tbuf_bytes = 0;
+                dspmm_status_t s = hip_status(hipMalloc(&A->tbuf, tb));
+                if (s != DSPMM_STATUS_SUCCESS) return s;
+                A->tbuf_bytes = tb;
+            }
+            base = reinterpret_cast<char *>(A->tbuf);
+        }
+        const int64_t ld = transposed_ld(nv);
+        const size_t bbytes = ((size_t)A->nc * (size_t)ld * (bf16 ? 2 : 4) + 255) / 256 * 256;
+        void *Bt = base;
+        float *Ct = reinterpret_cast<float *>(base + bbytes);
+        // column-major B is nv rows of nc (stride ldb); B^T is nc rows of ld
+        dspmm_status_t s = launch_transpose(B, ldb, nv, A->nc, Bt, ld, bf16 ? 2 : 4, st);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = run_ex(A, DENSE_ROW_MAJOR, nv, Bt, ld, bt, Ct, ld, cfg,
+                       ws ? base + tb : nullptr, ws ? ws_bytes - tb : 0, st, mid);
+        if (s == DSPMM_STATUS_SUCCESS) s = launch_transpose(Ct, ld, A->nr, nv, C, ldc, 4, st);
+        return s;
+    }
     switch (cfg.variant) {
     case DSPMM_VARIANT_NNZ_TILED: {
         if (layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
@@ -696,7 +756,6 @@ dspmm_status_t check_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const voi
     if (!A || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
     if (layout != DENSE_ROW_MAJOR && layout != DENSE_COL_MAJOR) return DSPMM_STATUS_INVALID_VALUE;
     if (bt != DSPMM_R_32F && bt != DSPMM_R_16BF) return DSPMM_STATUS_INVALID_VALUE;
-    if (bt == DSPMM_R_16BF && layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
     if (layout == DENSE_ROW_MAJOR) {
         if (ldb < nv || ldc < nv) return DSPMM_STATUS_INVALID_VALUE;
     } else {
@@ -716,6 +775,36 @@ dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_
     if (learned && layout == DENSE_ROW_MAJOR && bt == DSPMM_R_32F && learned_select(A, nv, &c))
         return c;
     return heuristic(A, layout, nv, bt);
+}
+
+// Workspace of a resolved configuration (caller holds A->mu).
+dspmm_status_t ex_workspace_bytes(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t btype,
+                                  dspmm_config c, size_t *bytes) {
+    *bytes = 0;
+    const bool bf16 = btype == DSPMM_R_16BF;
+    if (layout == DENSE_COL_MAJOR && rm_native(c, bf16)) {
+        size_t inner = 0;
+        dspmm_status_t st = ex_workspace_bytes(A, DENSE_ROW_MAJOR, nv, btype, c, &inner);
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        *bytes = (nv > 1 ? transposed_bytes(A, nv, bf16) : 0) + inner;
+        return DSPMM_STATUS_SUCCESS;
+    }
+    if (c.variant == DSPMM_VARIANT_SPMV_TILED && nv == 1) {
+        NbPlan *plan = nullptr;
+        dspmm_status_t st = get_plan(A, c.tile_nnz > 0 ? c.tile_nnz : kSpmvTile, &plan, kPlanSpmv);
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(1);
+        return DSPMM_STATUS_SUCCESS;
+    }
+    if (c.variant != DSPMM_VARIANT_NNZ_TILED) return DSPMM_STATUS_SUCCESS;
+    // Tile size does not depend on pointers; use aligned placeholders.
+    complete_nb(c, nv, nv, nv, reinterpret_cast<const void *>(256), reinterpret_cast<const void *>(256),
+                btype == DSPMM_R_16BF);
+    NbPlan *plan = nullptr;
+    dspmm_status_t st = get_plan(A, c.tile_nnz, &plan);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(nv);
+    return DSPMM_STATUS_SUCCESS;
 }
 
 float median_of(std::vector<float> v) {
@@ -849,26 +938,8 @@ dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout
                                             size_t *bytes) {
     if (!A || !bytes || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
     std::lock_guard<std::mutex> lk(A->mu);
-    dspmm_config c = resolve(A, layout, nv, btype, cfg);
-    *bytes = 0;
-    if (c.variant == DSPMM_VARIANT_SPMV_TILED && nv == 1) {
-        NbPlan *plan = nullptr;
-        dspmm_status_t st = get_plan(A, c.tile_nnz > 0 ? c.tile_nnz : kSpmvTile, &plan, kPlanSpmv);
-        if (st != DSPMM_STATUS_SUCCESS) return st;
-        *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(1);
-        return DSPMM_STATUS_SUCCESS;
-    }
-    if (c.variant != DSPMM_VARIANT_NNZ_TILED) return DSPMM_STATUS_SUCCESS;
-    // Tile size does not depend on pointers; use aligned placeholders.
-    complete_nb(c, nv, nv, nv, reinterpret_cast<const void *>(256), reinterpret_cast<const void *>(256),
-                btype == DSPMM_R_16BF);
-    NbPlan *plan = nullptr;
-    dspmm_status_t st = get_plan(A, c.tile_nnz, &plan);
-    if (st != DSPMM_STATUS_SUCCESS) return st;
-    *bytes = sizeof(float) * (size_t)plan->nslots * (size_t)carry_ld(nv);
-    return DSPMM_STATUS_SUCCESS;
+    return ex_workspace_bytes(A, layout, nv, btype, resolve(A, layout, nv, btype, cfg), bytes);
 }
-
 dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
                              int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,
                              const dspmm_config *cfg, void *workspace, size_t workspace_bytes,
@@ -955,8 +1026,10 @@ namespace {
 // Candidate list of the autotuner: the native nnz-tiled kernel over vector
 // widths (each with the lane count that covers nv in one slab), tile sizes
 // around the default and both unrolls; the row-sequential kernel over its
-// unrolls; the row kernels also in column slabs; the SpMV tiles when
-// nv == 1; then the four reference algorithm classes (FP32 B only).
+// unrolls; the row kernels also in column slabs; for column-major
+// operands, the nnz-tiled and row-sequential kernels on transposed copies;
+// the SpMV tiles when nv == 1; then the four reference algorithm classes
+// (FP32 B only).
 std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout layout, int nv,
                                               int64_t ldb, int64_t ldc, const void *B,
                                               const float *C, bool bf16) {
@@ -1052,6 +1125,25 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                 c.lanes = lanes;
                 cand.push_back(c);
             }
+    }
+    if (layout == DENSE_COL_MAJOR && nv > 1) {
+        // the row-major nnz-tiled and row-sequential kernels on transposed operands
+        const int64_t ld = transposed_ld(nv);
+        const void *al = reinterpret_cast<const void *>(256);
+        dspmm_config base{};
+        base.variant = DSPMM_VARIANT_NNZ_TILED;
+        complete_nb(base, nv, ld, ld, al, al, bf16);
+        const int s0 = default_tile(base.lanes);
+        for (int S : {s0 / 2, s0, s0 * 2}) {
+            if (S < 8 || nnz_tiled_lds_bytes(base.lanes, S) > kMaxTileLds) continue;
+            dspmm_config x = base;
+            x.tile_nnz = S;
+            cand.push_back(x);
+        }
+        dspmm_config r{};
+        r.variant = DSPMM_VARIANT_CSR_SCALAR;
+        r.unroll = 4;
+        cand.push_back(r);
     }
     if (nv == 1) {
         for (int T : {512, 1024, 2048, 4096}) {

@@ -77,9 +77,11 @@ struct dspmm_matrix_s {
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     float *carry = nullptr;
     size_t carry_bytes = 0;
+    void *tbuf = nullptr;       // column-major operands transposed for the row-major kernels
+    size_t tbuf_bytes = 0;
 
     size_t hbm_bytes() {
-        size_t b = base_bytes + carry_bytes;
+        size_t b = base_bytes + carry_bytes + tbuf_bytes;
         if (rowIdx) b += sizeof(int) * (size_t)nnz;
         for (auto &kv : plans) b += kv.second.bytes;
         return b;

@@ -132,6 +132,9 @@ bool spmv_tile_ok(int T);          // SpMV tile sizes with a kernel: 512 .. 4096
 size_t nnz_tiled_lds_bytes(int lanes, int S);
 int item_class(int vw, int lanes);
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+// dst[c * ldd + r] = src[r * lds + c] for r < rows, c < cols (2- or 4-byte elements).
+dspmm_status_t launch_transpose(const void *src, int64_t lds, int rows, int cols, void *dst,
+                                int64_t ldd, int elem_bytes, hipStream_t st);
 // Seeded uniform [-1, 1) fill of n FP32 (or BF16) values.
 dspmm_status_t launch_fill_uniform(void *out, int64_t n, uint64_t seed, bool bf16, hipStream_t st);
 dspmm_status_t launch_f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t st);

@@ -1083,6 +1083,44 @@ dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
     return s;
 }
 
+// dst[c * ldd + r] = src[r * lds + c] for r < rows, c < cols: 32 x 32 tiles
+// through LDS (one pad column against bank conflicts), one tile per
+// workgroup, tiles numbered row-major over the source.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_transpose(const T *__restrict__ src, int64_t lds, int rows,
+                                                      int cols, T *__restrict__ dst, int64_t ldd) {
+    __shared__ T tile[32][33];
+    const int tiles_c = (cols + 31) / 32;
+    const int tr = (int)(blockIdx.x / (unsigned)tiles_c), tc = (int)(blockIdx.x % (unsigned)tiles_c);
+    const int r0 = tr * 32, c0 = tc * 32;
+    const int tx = (int)threadIdx.x % 32, ty = (int)threadIdx.x / 32;
+    for (int k = ty; k < 32; k += kBlock / 32) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = src[(int64_t)r * lds + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += kBlock / 32) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[(int64_t)c * ldd + r] = tile[tx][k];
+    }
+}
+
+dspmm_status_t launch_transpose(const void *src, int64_t lds, int rows, int cols, void *dst,
+                                int64_t ldd, int elem_bytes, hipStream_t st) {
+    if (rows == 0 || cols == 0) return DSPMM_STATUS_SUCCESS;
+    const int64_t tiles = (int64_t)((rows + 31) / 32) * ((cols + 31) / 32);
+    if (tiles > INT_MAX) return DSPMM_STATUS_NOT_SUPPORTED;
+    if (elem_bytes == 2)
+        hipLaunchKernelGGL(k_transpose<uint16_t>, dim3((unsigned)tiles), dim3(kBlock), 0, st,
+                           reinterpret_cast<const uint16_t *>(src), lds, rows, cols,
+                           reinterpret_cast<uint16_t *>(dst), ldd);
+    else
+        hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)tiles), dim3(kBlock), 0, st,
+                           reinterpret_cast<const float *>(src), lds, rows, cols,
+                           reinterpret_cast<float *>(dst), ldd);
+    return launch_status();
+}
+
 dspmm_status_t launch_fill_zero(void *p, size_t bytes, hipStream_t st) {
     if (bytes == 0) return DSPMM_STATUS_SUCCESS;
     if ((bytes % 4) == 0 && aligned(p, 4)) {

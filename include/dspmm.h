@@ -278,9 +278,13 @@ typedef struct dspmm_config {
 
 /* C = A * B, overwriting C (nv columns; ldb/ldc in elements, >= nv for
  * row-major, >= nc / nr for column-major).  btype selects FP32 or BF16 B
- * storage (BF16: row-major only); accumulation is always FP32.  cfg may be
- * NULL (auto).  `workspace` may be NULL: the handle's own split-row buffer
- * is then used, so calls on one handle must then not overlap in time. */
+ * storage; accumulation is always FP32.  cfg may be NULL (auto).  The
+ * row-major kernels (NNZ_TILED, ROW_BLOCKED, NNZ_CACHED and the
+ * row-sequential CSR_SCALAR) take column-major operands through transposed
+ * copies of B and C (results equal the row-major call's bit for bit); the
+ * reference classes run column-major natively, FP32 only.  `workspace` may
+ * be NULL: the handle's own buffers (split rows, transposed copies) are
+ * then used, so calls on one handle must then not overlap in time. */
 DSPMM_API dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv,
                                        const void *B, int64_t ldb, dspmm_dtype_t btype,
                                        float *C, int64_t ldc, const dspmm_config *cfg,

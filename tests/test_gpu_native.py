@@ -289,9 +289,11 @@ def test_learned_selector_resolves_and_runs():
             out = run_native(c, m, nv)
             h32, c64, ab = c.ref(RM, nv)
             assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), f"selector {name} nv={nv}", h32)
-        # BF16 and column-major keep the static default
+        # BF16 and column-major keep the static default; column-major work of
+        # at least 2^20 (nnz x nv) goes to the nnz-tiled kernel on transposed copies
         assert m.resolve(8, btype=D.DSPMM_R_16BF).as_dict()["variant"] == "nnz_tiled"
-        assert m.resolve(8, layout=CM).as_dict()["variant"] == "csr_scalar"
+        want = "nnz_tiled" if c.nnz * 8 >= 1 << 20 else "csr_scalar"
+        assert m.resolve(8, layout=CM).as_dict()["variant"] == want
         m.close()
     note(f"selector picks: {picks}")
     assert any(p["variant"] != "nnz_tiled" for p in picks.values())
@@ -383,3 +385,43 @@ def test_row_sequential_leading_dimensions_and_bad_unroll():
         m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc,
                config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=5))
     assert e.value.status == D.STATUS_NOT_SUPPORTED
+
+
+@pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20"])
+def test_column_major_through_transposes(name):
+    """Row-major kernels on column-major operands run on transposed copies of
+    B and C: every output equals the row-major call's bit for bit (FP32 and
+    BF16 B), with padded leading dimensions, through the handle's scratch or
+    a caller workspace, and entries beyond the leading rows are untouched."""
+    c, m = matrix(name)
+    cfgs = [D.make_config(D.VARIANT_NNZ_TILED), D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
+            D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4)]
+    for nv in (3, 8, 64):
+        if nv > c.maxnv:
+            continue
+        Brm = c.dense(nv).reshape(c.nc, nv)
+        ldb, ldc = c.nc + 5, c.nr + 3
+        Bcm = np.zeros((nv, ldb), np.float32)
+        Bcm[:, :c.nc] = Brm.T
+        d_Bcm = D.DeviceArray.from_numpy(Bcm)
+        for bt in (D.DSPMM_R_32F, D.DSPMM_R_16BF):
+            bf = bt == D.DSPMM_R_16BF
+            d_Bcm_t = D.to_bf16_device(d_Bcm) if bf else d_Bcm
+            d_Brm_t = D.to_bf16_device(c.d_B) if bf else c.d_B
+            for cfg in cfgs:
+                what = f"{name} CM nv={nv} {'bf16' if bf else 'fp32'} {cfg.as_dict()['variant']}"
+                note(what)
+                ref = run_native(c, m, nv, config=cfg, btype=bt, B=d_Brm_t).reshape(c.nr, nv)
+                for use_ws in (False, True):
+                    d_C = D.DeviceArray.from_numpy(np.full((nv, ldc), -7.0, np.float32))
+                    ws, wsb = None, 0
+                    if use_ws:
+                        wsb = m.workspace_size(nv, layout=CM, btype=bt, config=cfg)
+                        assert wsb >= 4 * c.nr * nv, "the transposed copies need workspace"
+                        ws = D.DeviceArray.zeros(wsb // 4 + 64)
+                    m.spmm(d_Bcm_t, d_C, nv, layout=CM, ldb=ldb, ldc=ldc, btype=bt, config=cfg,
+                           workspace=ws, workspace_bytes=wsb)
+                    D.synchronize()
+                    out = d_C.numpy()
+                    assert np.all(out[:, c.nr:] == -7.0), what + ": padding written"
+                    assert np.array_equal(bits(out[:, :c.nr].T), bits(ref)), what
