@@ -1026,27 +1026,36 @@ namespace {
 // Candidate list of the autotuner: the native nnz-tiled kernel over vector
 // widths (each with the lane count that covers nv in one slab), tile sizes
 // around the default and both unrolls; the row-sequential kernel over its
-// unrolls; the row kernels also in column slabs; for column-major
-// operands, the nnz-tiled and row-sequential kernels on transposed copies;
-// the SpMV tiles when nv == 1; then the four reference algorithm classes
-// (FP32 B only).
+// unrolls; the row kernels also in column slabs (all of these also for
+// column-major operands, on transposed copies); the SpMV tiles when
+// nv == 1; then the four reference algorithm classes (FP32 B only).
 std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout layout, int nv,
                                               int64_t ldb, int64_t ldc, const void *B,
                                               const float *C, bool bf16) {
     std::vector<dspmm_config> cand;
+    // Column-major operands run the row-major kernels on transposed copies
+    // (leading dimension transposed_ld, 256-byte aligned), so every row-major
+    // candidate applies with those operands.
+    const bool cm = layout == DENSE_COL_MAJOR;
+    const void *al = reinterpret_cast<const void *>(256);
+    if (cm) {
+        ldb = ldc = nv > 1 ? transposed_ld(nv) : 1;
+        B = al;
+        C = reinterpret_cast<const float *>(al);
+    }
     // Lane counts of the nnz-tiled and row-sequential kernels: one slab
     // covering nv (0 = auto), and half / quarter slabs, which run the columns
     // in passes (grid.y is the slab, so one pass over all rows ends before
     // the next begins) and shrink the B footprint a pass keeps in L2 at the
     // cost of re-reading A.
     std::vector<int> slab_lanes{0};
-    if (layout == DENSE_ROW_MAJOR && nv > 1) {
+    if (nv > 1) {
         dspmm_config base{};
         base.variant = DSPMM_VARIANT_NNZ_TILED;
         complete_nb(base, nv, ldb, ldc, B, C, bf16);
         for (int l = base.lanes / 2; l >= 1 && l * 4 >= base.lanes; l /= 2) slab_lanes.push_back(l);
     }
-    if (layout == DENSE_ROW_MAJOR && nv > 1) {
+    if (nv > 1) {
         // column-cached tiles, only where neighbouring rows share columns
         double reuse;
         {
@@ -1081,7 +1090,7 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
             }
         }
     }
-    if (layout == DENSE_ROW_MAJOR) {
+    {
         dspmm_config base{};
         base.variant = DSPMM_VARIANT_NNZ_TILED;
         complete_nb(base, nv, ldb, ldc, B, C, bf16);
@@ -1114,7 +1123,7 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
             }
         }
     }
-    if (layout == DENSE_ROW_MAJOR) {
+    {
         // row-sequential (LDS-staged indices), over the gathers in flight
         for (int u : {2, 4, 8, 16})
             for (int lanes : slab_lanes) {
@@ -1125,25 +1134,6 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                 c.lanes = lanes;
                 cand.push_back(c);
             }
-    }
-    if (layout == DENSE_COL_MAJOR && nv > 1) {
-        // the row-major nnz-tiled and row-sequential kernels on transposed operands
-        const int64_t ld = transposed_ld(nv);
-        const void *al = reinterpret_cast<const void *>(256);
-        dspmm_config base{};
-        base.variant = DSPMM_VARIANT_NNZ_TILED;
-        complete_nb(base, nv, ld, ld, al, al, bf16);
-        const int s0 = default_tile(base.lanes);
-        for (int S : {s0 / 2, s0, s0 * 2}) {
-            if (S < 8 || nnz_tiled_lds_bytes(base.lanes, S) > kMaxTileLds) continue;
-            dspmm_config x = base;
-            x.tile_nnz = S;
-            cand.push_back(x);
-        }
-        dspmm_config r{};
-        r.variant = DSPMM_VARIANT_CSR_SCALAR;
-        r.unroll = 4;
-        cand.push_back(r);
     }
     if (nv == 1) {
         for (int T : {512, 1024, 2048, 4096}) {
