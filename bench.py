@@ -303,8 +303,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: the SpMM path here runs on a single GPU and "
-              f"--gpus {args.gpus} is not supported (DESIGN.md describes how it would shard "
-              f"by rows).", file=sys.stderr)
+              f"--gpus {args.gpus} is not supported (across GPUs the path would shard A by "
+              f"nonzero-balanced row ranges with B replicated).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
