@@ -174,7 +174,7 @@ def bench_stencil(D, W, np, warmup, steps, n=100, nv=64):
 
 
 def bench_batched(D, W, np, count, reps=20):
-    """Config 4: `count` independent matrices (M=K ~ U{256..16384}, LogNormal
+    """The batch workload: `count` independent matrices (M=K ~ U{256..16384}, LogNormal
     mean row length, N ~ U{16..256}), each autotuned, then run (i) as one
     persistent launch, (ii) as a replayed hipGraph of it, (iii) as one plain
     launch per matrix with the batch's configuration and (iv) with each
@@ -219,8 +219,8 @@ def bench_batched(D, W, np, count, reps=20):
 
 
 def bench_serving(D, W, np, count, n_req, n_large, rate=2000.0, p_large=0.05):
-    """Config 5: a seeded Poisson stream (`rate` req/s) drawing 95% of the
-    requests from the c4 pool (`count` matrices, each with its N) and 5% from
+    """The serving workload: a seeded Poisson stream (`rate` req/s) drawing 95% of the
+    requests from the batch workload's pool (`count` matrices, each with its N) and 5% from
     `n_large` large power-law matrices (M=K=2e6, ~40M nnz, N=64).  B is
     generated on the device per request (seeded by the request id).  The
     same stream is replayed against (i) the continuous-batching server,

@@ -7,7 +7,7 @@ Accuracy gates (the tolerance the tests apply):
     fit one tile equal the reference host loop bit for bit;
   * FP32: |c - c64| <= max(1e-5, gamma_{n+1}) * sum_p |a_p b_p| per output,
     where n is the row length and gamma_k = k u / (1 - k u), u = 2^-24.
-    This is the north star's 1e-5 relative error measured against the
+    This is the required 1e-5 relative error measured against the
     output's condition scale (|c64| alone is meaningless under
     cancellation); for rows longer than ~160 nonzeros 1e-5 is below the
     worst-case error of ANY FP32 summation order, including the

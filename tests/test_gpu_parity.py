@@ -62,7 +62,7 @@ def test_golden_all_algorithms(path, layout, alg):
 @pytest.mark.parametrize("layout", LAYOUTS, ids=["RM", "CM"])
 @pytest.mark.parametrize("alg", ALGS, ids=["CSR_SCALAR", "CSR_VECTOR", "COO_SCALAR", "COO_VECTOR"])
 def test_c1_sweep_all_nv(layout, alg):
-    """Config 1 (M=K=4096, 16 nnz/row): nv = 1..128 as the reference sweeps it."""
+    """The uniform matrix (M=K=4096, 16 nnz/row): nv = 1..128 as the reference sweeps it."""
     c = case("c1")
     for nv in (1, 2, 4, 8, 16, 32, 64, 128):
         run_and_check(c, layout, nv, alg)
@@ -91,7 +91,7 @@ def test_integer_operands_exact(layout, alg):
 @pytest.mark.parametrize("nv", [1, 32, 128])
 @pytest.mark.parametrize("alg", ALGS, ids=["CSR_SCALAR", "CSR_VECTOR", "COO_SCALAR", "COO_VECTOR"])
 def test_c2_row_major(nv, alg):
-    """Config 2 (M=K=262144, Zipf rows up to 5000): SpMV and SpMM."""
+    """The power-law matrix (M=K=262144, Zipf rows up to 5000): SpMV and SpMM."""
     run_and_check(case("c2"), D.DENSE_ROW_MAJOR, nv, alg)
 
 
