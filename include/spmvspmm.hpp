@@ -16,6 +16,8 @@
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <cstdlib>
+#include <ctime>
 #include <vector>
 
 #include "dspmm.h"
@@ -32,6 +34,24 @@ inline void check(dspmm_status_t s, const char *what) {
     if (s != DSPMM_STATUS_SUCCESS) throw Error(s, what);
 }
 
+// hgpu::InitGen (helper.h:42-76): the C library's rand() seeded once with
+// srand(seed) (time(0) by default); int values in [-50, 49], float values
+// in [-0.5, 0.499] in steps of 1/1000.
+template <typename T>
+struct InitGen;
+template <>
+struct InitGen<int> {
+    InitGen() : InitGen((unsigned)std::time(nullptr)) {}
+    explicit InitGen(unsigned seed) { std::srand(seed); }
+    int gen_next() { return std::rand() % 100 - 50; }
+};
+template <>
+struct InitGen<float> {
+    InitGen() : InitGen((unsigned)std::time(nullptr)) {}
+    explicit InitGen(unsigned seed) { std::srand(seed); }
+    float gen_next() { return (float)(std::rand() % 1000 - 500) / 1000; }
+};
+
 // hgpu::CUArray (helper.h:78-154): host vector + device buffer.
 template <typename T>
 class CUArray {
@@ -43,6 +63,17 @@ class CUArray {
     CUArray(CUArray &&o) noexcept : host_(std::move(o.host_)), dev_(o.dev_) { o.dev_ = nullptr; }
     ~CUArray() { free_(); }
 
+    // CUArray::init_random (helper.h:107-116): n values from InitGen<T>.
+    void init_random(size_t n) {
+        InitGen<T> gen;
+        host_.resize(n);
+        for (size_t i = 0; i < n; i++) host_[i] = gen.gen_next();
+    }
+    void init_random(size_t n, unsigned seed) {
+        InitGen<T> gen(seed);
+        host_.resize(n);
+        for (size_t i = 0; i < n; i++) host_[i] = gen.gen_next();
+    }
     void init_zeros(size_t n) { host_.assign(n, T(0)); }
     void assign(const std::vector<T> &data) { free_(); host_ = data; }
     // CUArray::sync_device (helper.h:135-140): (re)allocate and upload.

@@ -1,6 +1,7 @@
 // Compiles and links a C++ caller of the C-ABI and the SpmvSpmmProblem
 // mirror (include/spmvspmm.hpp); runs only calls that need no GPU.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "spmvspmm.hpp"
@@ -14,6 +15,19 @@ int main() {
                    nullptr, nullptr, nullptr, nullptr) != DSPMM_STATUS_NOT_SUPPORTED)
         return 2;
     if (std::strcmp(dspmm_get_error_string(DSPMM_STATUS_SUCCESS), "success") != 0) return 3;
+    // CUArray::init_random: the reference's generator, host side only
+    dspmm::CUArray<float> f;
+    f.init_random(1000, 7);
+    dspmm::CUArray<int> n;
+    n.init_random(1000, 7);
+    for (size_t i = 0; i < 1000; i++) {
+        const float v = f.get_host_array()[i];
+        const int k = n.get_host_array()[i];
+        if (v < -0.5f || v > 0.499f || k < -50 || k > 49) return 4;
+    }
+    std::srand(7);
+    const float first = (float)(std::rand() % 1000 - 500) / 1000;
+    if (f.get_host_array()[0] != first) return 5;
     std::printf("ok %s\n", dspmm_version());
     return 0;
 }
