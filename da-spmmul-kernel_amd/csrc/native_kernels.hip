@@ -1,6 +1,7 @@
-// native_kernels.hip -- the native load-balanced SpMM path: single-matrix
-// and batched (persistent) nnz-tiled kernels, the split-row combine, and
-// utility kernels.
+// native_kernels.hip -- the native SpMM path: single-matrix and batched
+// (persistent) nnz-tiled kernels, the split-row combine, the SpMV tiles,
+// the column-cached, row-blocked and row-sequential kernels, and utility
+// kernels (fills, BF16 conversion, transposes).
 //
 // Nnz-tiled SpMM (row-major, overwrite, deterministic).  A host-built plan
 // cuts the matrix into tiles of at most S nonzeros that start and end on
