@@ -46,6 +46,11 @@ void build_tiles(const std::vector<int> &rp, int nr, int S, std::vector<int4> &t
             r = re;
         }
     }
+    // Split rows with the most chunks first: their combine is the longest,
+    // so it starts first.  A row's own combine order does not depend on
+    // its place in this list.
+    std::stable_sort(splits.begin(), splits.end(),
+                     [](const int4 &a, const int4 &b) { return a.w > b.w; });
 }
 
 void build_spmv_tiles(const std::vector<int> &rp, int nr, int T, int row_max,
