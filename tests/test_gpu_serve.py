@@ -187,3 +187,36 @@ def test_large_requests_run_alone_with_their_own_kernel():
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
         assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"mixed request {k}")
     srv.close()
+
+
+def test_server_bf16_operands():
+    """BF16-stored B (generated on the device, rounded to nearest even): each
+    result is the same alone and in a burst, within the FP32 gate of the FP64
+    product on the rounded B, and within 1e-2 of the FP64 product on the
+    FP32 B."""
+    pool = make_pool(5, seed=64)
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16,
+                   btype=D.DSPMM_R_16BF)
+    mids = register(srv, pool)
+    alone = {}
+    for k in range(len(pool)):
+        assert srv.submit(srv.request(k, mids[k], pool[k][1], 500 + k)) == 0
+        srv.drain()
+        alone.update(results(srv, srv.poll(), pool, {k: k}))
+    reqs = [srv.request(100 + k, mids[k % len(pool)], pool[k % len(pool)][1], 500 + k % len(pool))
+            for k in range(4 * len(pool))]
+    srv.replay(reqs, [0.0] * len(reqs))
+    comps = srv.poll()
+    assert max(c["batch_size"] for c in comps) > 1
+    got = results(srv, comps, pool, {100 + k: k % len(pool) for k in range(len(reqs))})
+    for k in range(len(reqs)):
+        assert np.array_equal(bits(got[100 + k]), bits(alone[k % len(pool)])), f"request {k}"
+    for k, (a, nv) in enumerate(pool):
+        B = D.seeded_uniform(500 + k, a.nc * nv)
+        rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
+        c64r, abr = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, O.round_bf16(B))
+        assert_fp32(alone[k], c64r.reshape(-1), abr.reshape(-1), rl, f"bf16 request {k}")
+        c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
+        err = np.abs(alone[k].astype(np.float64) - c64.reshape(-1))
+        assert np.all(err <= 1e-2 * ab.reshape(-1)), f"bf16 request {k}: beyond 1e-2"
+    srv.close()
