@@ -426,3 +426,39 @@ def test_column_major_through_transposes(name):
                     out = d_C.numpy()
                     assert np.all(out[:, c.nr:] == -7.0), what + ": padding written"
                     assert np.array_equal(bits(out[:, :c.nr].T), bits(ref)), what
+
+
+def test_native_variants_on_empty_and_one_entry_matrices():
+    """Every native kernel overwrites C on a matrix with no nonzeros (zeros
+    everywhere, both layouts) and gets a one-nonzero matrix exactly right."""
+    nr, nc = 300, 50
+    empty = D.Matrix(nr, nc, np.zeros(nr + 1, np.int32), np.zeros(0, np.int32),
+                     np.zeros(0, np.float32))
+    rp = np.zeros(nr + 1, np.int32)
+    rp[8:] = 1                                      # row 7 holds the one nonzero
+    one = D.Matrix(nr, nc, rp, np.array([13], np.int32), np.array([2.5], np.float32))
+    cfgs = [D.make_config(D.VARIANT_NNZ_TILED), D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
+            D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2), D.make_config(D.VARIANT_NNZ_CACHED),
+            D.make_config(D.VARIANT_SPMV_TILED), None]
+    for nv in (1, 4):
+        B = W.gen_dense(5, nc * nv).reshape(nc, nv)
+        d_B = D.DeviceArray.from_numpy(B)
+        for layout in (RM, CM):
+            Bl = B if layout == RM else np.ascontiguousarray(B.T)
+            d_Bl = D.DeviceArray.from_numpy(Bl)
+            want = np.zeros((nr, nv), np.float32)
+            want[7] = 2.5 * B[13]
+            for cfg in cfgs:
+                if cfg is not None and cfg.variant == D.VARIANT_SPMV_TILED and nv != 1:
+                    continue
+                what = f"nv={nv} layout={layout} {cfg.as_dict()['variant'] if cfg else 'auto'}"
+                for m, ref in ((empty, np.zeros((nr, nv), np.float32)), (one, want)):
+                    d_C = D.DeviceArray.from_numpy(np.full(nr * nv, 7.0, np.float32))
+                    m.spmm(d_Bl, d_C, nv, layout=layout, config=cfg)
+                    D.synchronize()
+                    got = d_C.numpy().reshape((nr, nv) if layout == RM else (nv, nr))
+                    got = got if layout == RM else got.T
+                    assert np.array_equal(got, ref), what
+    del d_B
+    empty.close()
+    one.close()
