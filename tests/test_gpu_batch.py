@@ -134,3 +134,40 @@ def test_benchmark_modes_run():
         ms, per = b.benchmark(mode, warmup=1, reps=3)
         note(f"mode {mode}: {ms * 1e3:.1f} us")
         assert ms > 0 and len(per) == 3
+
+
+def test_batch_with_empty_and_tiny_matrices():
+    """A batch holding a matrix without nonzeros, a one-row one-nonzero
+    matrix and a matrix without rows beside ordinary ones: the empty one's C
+    is overwritten with zeros, the tiny one is exact, and every item equals
+    its single-matrix launch bit for bit."""
+    rng = np.random.default_rng(45)
+    mats = [W.Csr(300, 50, np.zeros(301, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32)),
+            W.Csr(1, 7, np.array([0, 1], np.int32), np.array([4], np.int32),
+                  np.array([-1.5], np.float32)),
+            W.Csr(0, 9, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))]
+    mats += [W.gen_csr(W.GEN_BINOMIAL, 700, 700, 4500 + k, p0=12.0) for k in range(3)]
+    items = []
+    for k, a in enumerate(mats):
+        nv = int(rng.choice([1, 5, 16]))
+        it = Item(k, a, nv, 4600 + k)
+        items.append(it)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    b.run()
+    D.synchronize()
+    for it in items:
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=D.make_config(D.VARIANT_NNZ_TILED))
+    D.synchronize()
+    for i, it in enumerate(items):
+        if it.a.nr == 0:
+            continue
+        out = it.d_C.numpy().reshape(-1)
+        assert_bitwise(out, it.d_C_ref.numpy().reshape(-1), f"item {i} batched vs single")
+        h32 = O.spmm_host32(RM, it.a.nr, it.a.nc, it.nv, it.a.rowPtr, it.a.colIdx, it.a.values,
+                            it.B).reshape(-1)
+        if i < 2:
+            assert np.array_equal(bits(out), bits(h32)), f"item {i} vs host loop"
+        else:
+            c64, ab, rl = it.oracle()
+            assert_fp32(out, c64, ab, rl, f"item {i}", h32)
+    b.close()
