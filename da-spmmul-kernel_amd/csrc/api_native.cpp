@@ -229,9 +229,12 @@ dspmm_status_t ensure_rowidx(dspmm_matrix_s *A) {
 }
 
 bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
-                      const std::vector<float> &va, int nr, int R, std::vector<int> &boff,
-                      std::vector<int2> &list, std::vector<float> &vals) {
-    const int nb = (nr + R - 1) / R;
+                      const std::vector<float> &va, int nr, int R, int stride,
+                      std::vector<int> &boff, std::vector<int2> &list, std::vector<float> &vals) {
+    const int64_t period = (int64_t)R * stride;
+    const int64_t nb64 = (nr + period - 1) / period * stride;
+    if (stride < 1 || nb64 >= INT_MAX) return false;
+    const int nb = (int)nb64;
     boff.assign((size_t)nb + 1, 0);
     list.clear();
     vals.clear();
@@ -240,10 +243,11 @@ bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
             if (ci[(size_t)p] <= ci[(size_t)p - 1]) return false;
     int head[8], end[8];
     for (int b = 0; b < nb; b++) {
-        const int r0 = b * R, nrows = std::min(R, nr - r0);
+        const int64_t r0 = (int64_t)(b / stride) * period + b % stride;
+        const int nrows = r0 >= nr ? 0 : (int)std::min<int64_t>(R, (nr - 1 - r0) / stride + 1);
         for (int i = 0; i < nrows; i++) {
-            head[i] = rp[(size_t)r0 + i];
-            end[i] = rp[(size_t)r0 + i + 1];
+            head[i] = rp[(size_t)(r0 + (int64_t)i * stride)];
+            end[i] = rp[(size_t)(r0 + (int64_t)i * stride) + 1];
         }
         for (;;) {
             int c = INT_MAX;
@@ -303,8 +307,62 @@ double column_reuse(dspmm_matrix_s *A) {
     return A->reuse16;
 }
 
-dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind) {
-    const int key = 4 * S + kind;
+int shared_row_stride(dspmm_matrix_s *A) {
+    if (A->rb_stride >= 1) return A->rb_stride;
+    A->rb_stride = 1;
+    if (A->nnz == 0 || A->nr < 3) return A->rb_stride;
+    std::vector<int> ci;
+    if (host_colidx(A, ci) != DSPMM_STATUS_SUCCESS) return A->rb_stride;
+    const std::vector<int> &rp = A->h_rowPtr;
+    const int nsamp = std::min(256, A->nr);
+    std::vector<int> sample((size_t)nsamp);
+    for (int k = 0; k < nsamp; k++) sample[(size_t)k] = (int)((int64_t)k * (A->nr - 1) / std::max(1, nsamp - 1));
+    // candidates: the most frequent column-minus-row offsets >= 2
+    std::vector<int64_t> offs;
+    for (int r : sample)
+        for (int p = rp[(size_t)r]; p < rp[(size_t)r + 1]; p++) {
+            const int64_t d = (int64_t)ci[(size_t)p] - r;
+            if (d >= 2 && d < A->nr) offs.push_back(d);
+        }
+    std::sort(offs.begin(), offs.end());
+    std::vector<std::pair<int, int64_t>> freq;             // (-count, offset)
+    for (size_t i = 0; i < offs.size();) {
+        size_t j = i;
+        while (j < offs.size() && offs[j] == offs[i]) j++;
+        freq.emplace_back(-(int)(j - i), offs[i]);
+        i = j;
+    }
+    std::sort(freq.begin(), freq.end());
+    if (freq.size() > 16) freq.resize(16);
+    // score: columns a sampled row shares with the row that far below it
+    double best = 0;
+    int64_t best_d = 1;
+    for (const auto &f : freq) {
+        const int64_t d = f.second;
+        int64_t shared = 0, rows = 0;
+        for (int r : sample) {
+            if (r + d >= A->nr) continue;
+            rows++;
+            int p = rp[(size_t)r], q = rp[(size_t)(r + d)];
+            const int pe = rp[(size_t)r + 1], qe = rp[(size_t)(r + d) + 1];
+            while (p < pe && q < qe) {
+                if (ci[(size_t)p] < ci[(size_t)q]) p++;
+                else if (ci[(size_t)p] > ci[(size_t)q]) q++;
+                else { shared++; p++; q++; }
+            }
+        }
+        const double score = rows ? (double)shared / (double)rows : 0.0;
+        if (score > best || (score == best && score > 0 && d > best_d)) {
+            best = score;
+            best_d = d;
+        }
+    }
+    if (best >= std::max(1.0, 0.25 * (double)A->nnz / (double)A->nr)) A->rb_stride = (int)best_d;
+    return A->rb_stride;
+}
+
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int stride) {
+    const int64_t key = 4 * ((int64_t)S + 16 * (int64_t)(stride - 1)) + kind;
     auto it = A->plans.find(key);
     if (it != A->plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
     std::vector<int4> tiles, splits;
@@ -331,7 +389,7 @@ dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind) {
         if (s != DSPMM_STATUS_SUCCESS) return s;
         NbPlan p;
         p.S = S;
-        p.usable = build_row_blocks(A->h_rowPtr, ci, va, A->nr, S, boff, list, rvals);
+        p.usable = build_row_blocks(A->h_rowPtr, ci, va, A->nr, S, stride, boff, list, rvals);
         if (p.usable) {
             p.ntiles = (int)boff.size() - 1;
             p.reuse = list.empty() ? 1.0 : (double)A->nnz / (double)list.size();
@@ -617,8 +675,11 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         dspmm_config c = cfg;
         c.tile_nnz = 0;
         complete_nb(c, nv, ldb, ldc, B, C, bf16);
+        if (cfg.row_stride < 0 || cfg.row_stride > std::max(1, A->nr))
+            return DSPMM_STATUS_INVALID_VALUE;
+        const int stride = std::max(1, cfg.row_stride);
         NbPlan *plan = nullptr;
-        dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock);
+        dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock, stride);
         if (s != DSPMM_STATUS_SUCCESS) return s;
         if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
         const Slab sl = make_slab(nv, c.vec);
@@ -635,6 +696,7 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         a.nv = nv;
         a.nblocks = plan->ntiles;
         a.rows = R;
+        a.stride = stride;
         a.vw = c.vec;
         a.lanes = std::min(c.lanes, sl.lanes);
         a.slab_w = a.lanes * c.vec;
@@ -1067,16 +1129,24 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
             std::lock_guard<std::mutex> lk(A->mu);
             reuse = column_reuse(A);
         }
-        if (reuse >= kRowBlockMinReuse) {
+        int stride;
+        {
+            std::lock_guard<std::mutex> lk(A->mu);
+            stride = shared_row_stride(A);
+        }
+        std::vector<int> strides;
+        if (reuse >= kRowBlockMinReuse) strides.push_back(1);
+        if (stride > 1) strides.push_back(stride);
+        for (int rs : strides)
             for (int R : {2, 4, 8})
                 for (int u : {4, 8}) {
                     dspmm_config c{};
                     c.variant = DSPMM_VARIANT_ROW_BLOCKED;
                     c.tile_nnz = R;
                     c.unroll = u;
+                    c.row_stride = rs;
                     cand.push_back(c);
                 }
-        }
         if (reuse >= kCachedMinReuse) {
             dspmm_config base{};
             base.variant = DSPMM_VARIANT_NNZ_TILED;

@@ -55,8 +55,8 @@ void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, 
 // clear).  False if a row is unsorted or repeats a column (the plan then
 // cannot keep every row's nonzero order).
 bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
-                      const std::vector<float> &va, int nr, int R, std::vector<int> &boff,
-                      std::vector<int2> &list, std::vector<float> &vals);
+                      const std::vector<float> &va, int nr, int R, int stride,
+                      std::vector<int> &boff, std::vector<int2> &list, std::vector<float> &vals);
 
 enum PlanKind { kPlanNnzTiled = 0, kPlanSpmv = 1, kPlanCached = 2, kPlanRowBlock = 3 };
 constexpr size_t kMaxCachedLds = 80 * 1024;   // two column-cached workgroups per CU
@@ -72,8 +72,9 @@ struct dspmm_matrix_s {
     double mean = 0, cv = 0;
     size_t base_bytes = 0;
     std::mutex mu;
-    std::map<int, dspmm::NbPlan> plans;                            // by 4 * tile size + kind
+    std::map<int64_t, dspmm::NbPlan> plans;                        // by 4 * (tile size + 16 * (stride - 1)) + kind
     double reuse16 = -1;     // sampled nonzeros per distinct column over 16-row windows (-1: not yet)
+    int rb_stride = -1;      // row distance of the most shared columns, sampled (-1: not yet)
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     float *carry = nullptr;
     size_t carry_bytes = 0;
@@ -91,9 +92,16 @@ struct dspmm_matrix_s {
 namespace dspmm {
 
 // All of these expect the caller to hold A->mu.
-dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind = kPlanNnzTiled);
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind = kPlanNnzTiled,
+                        int stride = 1);
 // Column reuse of neighbouring rows, sampled (computed once per matrix).
 double column_reuse(dspmm_matrix_s *A);
+// The row distance >= 2 at which up to 256 sampled rows share the most
+// columns with the row that far below them (candidates: the 16 most
+// frequent column-minus-row offsets; ties go to the larger distance), or 1
+// when they share fewer than max(1, mean row length / 4) on average.
+// Computed once per matrix.
+int shared_row_stride(dspmm_matrix_s *A);
 int cached_smax(int umax);
 int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,

@@ -71,6 +71,7 @@ struct RowBlockLaunch {
     int64_t ldb = 0, ldc = 0;
     int nr = 0, nv = 0, nblocks = 0;
     int rows = 4;                      // R: 2, 4 or 8
+    int stride = 1;                    // row distance within a block
     int vw = 0, lanes = 0, slab_w = 0, nslabs = 1, unroll = 8;
     bool bf16 = false;
 };

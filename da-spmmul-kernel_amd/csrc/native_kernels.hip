@@ -575,8 +575,11 @@ __global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
 }
 
 // ---- row-blocked SpMM (row-major) -------------------------------------------
-// For matrices whose neighbouring rows share columns.  A host plan groups
-// the rows in blocks of R consecutive rows and lists, per block, the
+// For matrices whose rows share columns with the rows a fixed distance
+// away (stencils, banded matrices).  A host plan groups the rows in blocks
+// of R rows `stride` apart -- block b holds rows
+// (b / stride) * R * stride + b % stride + i * stride, i < R, so
+// consecutive blocks hold neighbouring rows -- and lists, per block, the
 // distinct columns of those rows in ascending order, each with a mask of
 // the block's rows that have it ({col, mask}), and the block's values in
 // that (column, row) order.  A group of L lanes x VW columns owns a block:
@@ -584,15 +587,17 @@ __global__ __launch_bounds__(NT) void k_nnz_cached(CachedLaunch a) {
 // rows in its mask.  Rows are sorted, so each row still sums its nonzeros
 // in nonzero order with a separately rounded multiply and add -- bit-exact
 // with the host loop -- while the B rows a block's rows share are gathered
-// once instead of once per row (a 27-point stencil: 9 (R + 2) gathers for
-// 27 R nonzeros).
+// once instead of once per row.  A 27-point stencil in blocks of R rows
+// one grid plane apart lists 9 (R + 2) columns for 27 R nonzeros, and the
+// next block (the neighbouring rows) finds two thirds of them in L1.
 //
-// A wavefront owns G = 64 / L consecutive row blocks.  Their list entries
-// and values are contiguous, so the wavefront stages them into its own LDS
-// region in windows of kRbWin entries with coalesced 16-byte loads; the
-// groups then read their (col, mask, values) from LDS as broadcasts.  The
-// only per-entry vector-memory instruction left is the B-row gather, which
-// is what the L1 address/data path is busy with.
+// A wavefront owns G = 64 / L consecutive row blocks.  Each group stages
+// its own block's list entries and values into its slice of the
+// wavefront's LDS region (kRbWin / G entries per window) with coalesced
+// loads, so all G groups work in every window; the groups then read their
+// (col, mask, values) from LDS as broadcasts.  The only per-entry
+// vector-memory instruction left is the B-row gather, which is what the
+// L1 address/data path is busy with.
 constexpr int kRbWin = 128;
 
 template <int R>
@@ -613,60 +618,62 @@ template <typename TB, int VW, int L, int R, int UNR>
 __global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
     constexpr int G = kWave / L;                 // row blocks per wavefront
     constexpr int WPB = kBlock / kWave;          // wavefronts per workgroup
+    constexpr int WG = kRbWin / G;               // list entries per group and window
     static_assert(R == 2 || R == 4 || R == 8, "R");
     __shared__ int2 s_list[WPB][kRbWin];
     __shared__ __align__(16) float s_vals[WPB][kRbWin * R];
-    const int t = threadIdx.x, wv = t / kWave, lane = t % kWave, g = lane / L;
+    const int t = threadIdx.x, wv = t / kWave, lane = t % kWave, g = lane / L, l = lane % L;
     const int blk_w = (xcd_remap(blockIdx.x, gridDim.x) * WPB + wv) * G;
     if (blk_w >= a.nblocks) return;              // the whole wavefront leaves
-    const int nb = min(G, a.nblocks - blk_w);
     const int blk = blk_w + g;
-    const int col0 = blockIdx.y * a.slab_w + (lane % L) * VW;
-    const bool active = g < nb && col0 < a.nv;
-    const int e0 = a.boff[blk_w], e1 = a.boff[blk_w + nb];
-    int u0 = 0, u1 = 0;
-    if (g < nb) {
+    const int col0 = blockIdx.y * a.slab_w + l * VW;
+    const bool has = blk < a.nblocks;
+    const bool active = has && col0 < a.nv;
+    int u0 = 0, len = 0;
+    if (has) {
         u0 = a.boff[blk];
-        u1 = a.boff[blk + 1];
+        len = a.boff[blk + 1] - u0;
     }
+    int wlen = len;                              // longest list of the wavefront
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) wlen = max(wlen, __shfl_xor(wlen, o));
     const TB *__restrict__ B = reinterpret_cast<const TB *>(a.B) + col0;
-    int2 *sl = s_list[wv];
-    float *sv = s_vals[wv];
+    int2 *sl = s_list[wv] + g * WG;
+    float *sv = s_vals[wv] + g * WG * R;
     float acc[R][VW];
 #pragma unroll
     for (int i = 0; i < R; i++)
 #pragma unroll
         for (int k = 0; k < VW; k++) acc[i][k] = 0.f;
-    for (int w = e0; w < e1; w += kRbWin) {
-        const int cnt = min(kRbWin, e1 - w);
+    for (int w = 0; w < wlen; w += WG) {
+        const int cnt = max(0, min(WG, len - w));
         wave_lds_sync();
-        for (int i = lane; i < cnt; i += kWave) sl[i] = a.list[w + i];
+        for (int i = l; i < cnt; i += L) sl[i] = a.list[u0 + w + i];
         if constexpr (R >= 4) {
-            const float4 *src = reinterpret_cast<const float4 *>(a.vals + (int64_t)w * R);
+            const float4 *src = reinterpret_cast<const float4 *>(a.vals + (int64_t)(u0 + w) * R);
             float4 *dst = reinterpret_cast<float4 *>(sv);
-            for (int i = lane; i < cnt * (R / 4); i += kWave) dst[i] = src[i];
+            for (int i = l; i < cnt * (R / 4); i += L) dst[i] = src[i];
         } else {
-            const float2 *src = reinterpret_cast<const float2 *>(a.vals + (int64_t)w * R);
+            const float2 *src = reinterpret_cast<const float2 *>(a.vals + (int64_t)(u0 + w) * R);
             float2 *dst = reinterpret_cast<float2 *>(sv);
-            for (int i = lane; i < cnt; i += kWave) dst[i] = src[i];
+            for (int i = l; i < cnt; i += L) dst[i] = src[i];
         }
         wave_lds_sync();
         if (!active) continue;
-        const int p0 = max(u0, w) - w, p1 = min(u1, w + cnt) - w;
-        for (int u = p0; u < p1; u += UNR) {
+        for (int u = 0; u < cnt; u += UNR) {
             int2 cm[UNR];
             float bv[UNR][VW];
 #pragma unroll
             for (int x = 0; x < UNR; x++) {
-                cm[x] = sl[min(u + x, p1 - 1)];          // padded: no branch around loads
-                if (u + x >= p1) cm[x].y = 0;
+                cm[x] = sl[min(u + x, cnt - 1)];         // padded: no branch around loads
+                if (u + x >= cnt) cm[x].y = 0;
             }
 #pragma unroll
             for (int x = 0; x < UNR; x++) load_b<TB, VW>(bv[x], B + (int64_t)cm[x].x * a.ldb);
 #pragma unroll
             for (int x = 0; x < UNR; x++) {
                 float v[R];
-                lds_row_vals<R>(v, sv + min(u + x, p1 - 1) * R);
+                lds_row_vals<R>(v, sv + min(u + x, cnt - 1) * R);
 #pragma unroll
                 for (int i = 0; i < R; i++) {
                     // a branch, not a select: groups of one wavefront usually
@@ -681,10 +688,12 @@ __global__ __launch_bounds__(kBlock) void k_row_blocked(RowBlockLaunch a) {
         }
     }
     if (!active) return;
-    const int r0 = blk * R;
+    const int64_t r0 = (int64_t)(blk / a.stride) * R * a.stride + blk % a.stride;
 #pragma unroll
-    for (int i = 0; i < R; i++)
-        if (r0 + i < a.nr) store_vec_nt<VW>(a.C + (int64_t)(r0 + i) * a.ldc + col0, acc[i]);
+    for (int i = 0; i < R; i++) {
+        const int64_t r = r0 + (int64_t)i * a.stride;
+        if (r < a.nr) store_vec_nt<VW>(a.C + r * a.ldc + col0, acc[i]);
+    }
 }
 
 __global__ void k_fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {

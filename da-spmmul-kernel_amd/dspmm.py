@@ -90,11 +90,12 @@ class MatrixDesc(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("variant", C.c_int), ("vec", C.c_int), ("lanes", C.c_int),
-                ("tile_nnz", C.c_int), ("unroll", C.c_int)]
+                ("tile_nnz", C.c_int), ("unroll", C.c_int), ("row_stride", C.c_int)]
 
     def as_dict(self):
         return {"variant": VARIANT_NAMES.get(self.variant, self.variant), "vec": self.vec,
-                "lanes": self.lanes, "tile_nnz": self.tile_nnz, "unroll": self.unroll}
+                "lanes": self.lanes, "tile_nnz": self.tile_nnz, "unroll": self.unroll,
+                "row_stride": self.row_stride}
 
 
 class BatchItem(C.Structure):
@@ -392,8 +393,8 @@ def launch(name, nr, nc, nnz, nv, rowPtr, rowIdx, colIdx, values, dnInput, dnOut
              ptr(dnOutput), stream)
 
 
-def make_config(variant=VARIANT_AUTO, vec=0, lanes=0, tile_nnz=0, unroll=0) -> Config:
-    return Config(variant, vec, lanes, tile_nnz, unroll)
+def make_config(variant=VARIANT_AUTO, vec=0, lanes=0, tile_nnz=0, unroll=0, row_stride=0) -> Config:
+    return Config(variant, vec, lanes, tile_nnz, unroll, row_stride)
 
 
 def config_from_dict(d) -> Config:
@@ -401,7 +402,8 @@ def config_from_dict(d) -> Config:
     names = {v: k for k, v in VARIANT_NAMES.items()}
     v = d["variant"]
     return make_config(names.get(v, v) if isinstance(v, str) else int(v), int(d.get("vec", 0)),
-                       int(d.get("lanes", 0)), int(d.get("tile_nnz", 0)), int(d.get("unroll", 0)))
+                       int(d.get("lanes", 0)), int(d.get("tile_nnz", 0)), int(d.get("unroll", 0)),
+                       int(d.get("row_stride", 0)))
 
 
 class Matrix:

@@ -37,6 +37,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--unroll", type=int, default=0)
+    ap.add_argument("--row-stride", type=int, default=0)
     args = ap.parse_args()
 
     import numpy as np
@@ -63,7 +64,8 @@ def main():
     D.synchronize()
 
     if args.variant:
-        cfg = D.make_config(args.variant, args.vec, args.lanes, args.tile, args.unroll)
+        cfg = D.make_config(args.variant, args.vec, args.lanes, args.tile, args.unroll,
+                            args.row_stride)
     else:
         cfg, _ = m.autotune(d_B, d_C, nv, btype=btype, reps=3)
     cfg = m.resolve(nv, btype=btype, config=cfg)

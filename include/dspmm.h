@@ -262,10 +262,11 @@ typedef enum dspmm_variant {
                                     0 = from the plan's rows per tile) */
     DSPMM_VARIANT_ROW_BLOCKED = 8  /* row-major, sorted rows without repeated
                                     columns: a lane group owns tile_nnz = 2, 4
-                                    or 8 consecutive rows and loads each B row
-                                    their columns share once; rows are summed
-                                    like the host loop (row-exact); unroll =
-                                    B rows in flight (4 or 8) */
+                                    or 8 rows, row_stride apart (consecutive
+                                    by default), and loads each B row their
+                                    columns share once; rows are summed like
+                                    the host loop (row-exact); unroll = B rows
+                                    in flight (4 or 8) */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {
@@ -274,6 +275,11 @@ typedef struct dspmm_config {
     int lanes;      /* lanes per row slab (power of two <= 64); 0 = auto */
     int tile_nnz;   /* NNZ_TILED: max nonzeros per tile; 0 = auto */
     int unroll;     /* NNZ_TILED: gathers in flight per lane group; 0 = auto */
+    int row_stride; /* ROW_BLOCKED: distance between the rows of a block, in
+                       [0, max(1, nr)]; 0 or 1 = consecutive rows.  Other
+                       variants ignore it.  The autotuner also tries the
+                       distance at which sampled rows share the most columns
+                       (a grid stencil's line or plane length). */
 } dspmm_config;
 
 /* C = A * B, overwriting C (nv columns; ldb/ldc in elements, >= nv for

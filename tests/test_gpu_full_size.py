@@ -56,8 +56,8 @@ def run(m, d_B, d_C, nv, cfg, btype=D.DSPMM_R_32F):
 
 def test_stencil_100_cubed_row_exact_kernels_agree():
     """27-point stencil on a 100^3 grid (M = 1e6, 26.5 M nonzeros), N = 64:
-    four row-exact kernels give the same 64 M outputs; sampled rows equal the
-    host loop; the same with BF16-stored B."""
+    five row-exact kernels give the same 64 M outputs; sampled rows equal the
+    host loop; the same with BF16-stored B (three kernels)."""
     a = W.c3_matrix(100)
     nv = 64
     B = W.dense_b(3, a.nc, nv)
@@ -67,7 +67,9 @@ def test_stencil_100_cubed_row_exact_kernels_agree():
     cfgs = {"reference-class SR_RB": D.make_config(D.VARIANT_CSR_SCALAR),
             "row-sequential": D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
             "nnz-tiled S=64": D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=64),
-            "row-blocked R=2": D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4)}
+            "row-blocked R=2": D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4),
+            "row-blocked R=4, rows a plane apart":
+                D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4, unroll=4, row_stride=10000)}
     first = None
     for name, cfg in cfgs.items():
         note(f"c3 full {name}")
@@ -82,8 +84,10 @@ def test_stencil_100_cubed_row_exact_kernels_agree():
     d16 = D.to_bf16_device(d_B)
     outs = [run(m, d16, d_C, nv, cfg, D.DSPMM_R_16BF) for cfg in
             (D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
-             D.make_config(D.VARIANT_NNZ_TILED, vec=8, lanes=8, tile_nnz=32, unroll=4))]
-    assert np.array_equal(bits(outs[0]), bits(outs[1])), "c3 BF16: row kernels differ"
+             D.make_config(D.VARIANT_NNZ_TILED, vec=8, lanes=8, tile_nnz=32, unroll=4),
+             D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4, row_stride=10000))]
+    for o in outs[1:]:
+        assert np.array_equal(bits(outs[0]), bits(o)), "c3 BF16: row kernels differ"
     rows, rp, ci, v = sample(a, 3000, 32)
     h32 = O.spmm_host32(RM, rows.size, a.nc, nv, rp, ci, v, O.round_bf16(B)).reshape(-1)
     got = np.ascontiguousarray(outs[0].reshape(a.nr, nv)[rows]).reshape(-1)

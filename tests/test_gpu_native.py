@@ -324,6 +324,51 @@ def test_row_blocked(name, R, unroll):
     assert_bitwise(out, h32r, f"{name} row-blocked R={R} bf16")
 
 
+@pytest.mark.parametrize("name,strides", [("stencil_20", (20, 400, 7, 8000)),
+                                          ("edge", (3, 1000, 3000)), ("zipf_mid", (5,)),
+                                          ("int_valued", (2,))])
+@pytest.mark.parametrize("R", [2, 4, 8])
+def test_row_blocked_strided(name, strides, R):
+    """Row blocks whose rows are row_stride apart, with strides that leave a
+    partial last period, blocks past the last row and a stride equal to the
+    row count: every output still equals the host loop bit for bit."""
+    c, m = matrix(name)
+    for rs in strides:
+        cfg = D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=R, unroll=4, row_stride=rs)
+        for nv in (1, 8, 64):
+            if nv > c.maxnv:
+                continue
+            h32, c64, ab = c.ref(RM, nv)
+            what = f"{name} row-blocked R={R} stride={rs} nv={nv}"
+            note(what)
+            assert_bitwise(run_native(c, m, nv, config=cfg), h32, what)
+    nv = min(64, c.maxnv)
+    d16 = D.to_bf16_device(c.d_B)
+    cfg = D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=R, unroll=8, row_stride=strides[0])
+    out = run_native(c, m, nv, config=cfg, btype=D.DSPMM_R_16BF, B=d16)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(c.dense(nv))).reshape(-1)
+    assert_bitwise(out, h32r, f"{name} row-blocked R={R} stride={strides[0]} bf16")
+
+
+def test_row_blocked_stride_bounds_and_autotune():
+    """row_stride outside [0, nr] is rejected; on a 20^3 stencil the
+    autotuner tries blocks of consecutive rows and blocks one plane (400
+    rows) apart: sampled rows share 16.8 columns on average with the row a
+    plane below, 16.0 with the row a line below, 11 or fewer at the other
+    candidate distances."""
+    c, m = matrix("stencil_20")
+    for bad in (-1, c.nr + 1):
+        with pytest.raises(D.DspmmError) as e:
+            m.spmm(c.d_B, c.d_C, 8, config=D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4,
+                                                         row_stride=bad))
+        assert e.value.status == D.STATUS_INVALID_VALUE
+    tab = m.autotune_table(c.d_B, c.d_C, 64, reps=2)
+    strided = {(cfg.tile_nnz, cfg.row_stride) for cfg, ms in tab
+               if cfg.as_dict()["variant"] == "row_blocked" and ms is not None}
+    assert {(R, s) for R in (2, 4, 8) for s in (1, 400)} <= strided, strided
+
+
 def test_row_blocked_rejects_repeated_columns():
     rp = np.array([0, 3, 4], np.int32)
     ci = np.array([1, 1, 2, 0], np.int32)
