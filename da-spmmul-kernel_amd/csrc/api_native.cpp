@@ -675,9 +675,9 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         dspmm_config c = cfg;
         c.tile_nnz = 0;
         complete_nb(c, nv, ldb, ldc, B, C, bf16);
-        if (cfg.row_stride < 0 || cfg.row_stride > std::max(1, A->nr))
+        if (cfg.row_stride < -1 || cfg.row_stride > std::max(1, A->nr))
             return DSPMM_STATUS_INVALID_VALUE;
-        const int stride = std::max(1, cfg.row_stride);
+        const int stride = cfg.row_stride < 0 ? shared_row_stride(A) : std::max(1, cfg.row_stride);
         NbPlan *plan = nullptr;
         dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock, stride);
         if (s != DSPMM_STATUS_SUCCESS) return s;

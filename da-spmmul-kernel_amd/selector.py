@@ -44,7 +44,7 @@ NVS = (1, 8, 32, 64, 128)
 TOP = 8            # configurations kept per corpus entry
 K = 3              # neighbours that vote
 FEATURES = ("log_rows", "log_mean", "cv", "log_skew", "reuse")
-CFG_FIELDS = ("variant", "vec", "lanes", "tile_nnz", "unroll")
+CFG_FIELDS = ("variant", "vec", "lanes", "tile_nnz", "unroll", "row_stride")
 
 
 def corpus():
@@ -109,8 +109,13 @@ def collect(out_path, reps=5):
             tab = m.autotune_table(d_B, d_C, nv, reps=reps)
             ok = [(c, ms) for c, ms in tab if ms is not None]
             for c, ms in ok:
-                rows.append({"matrix": name, "nv": nv, **f, "nnz": a.nnz,
-                             **{k: getattr(c, k) for k in CFG_FIELDS}, "ms": ms})
+                cfg = {k: getattr(c, k) for k in CFG_FIELDS}
+                # the only row stride above 1 the autotuner tries is the matrix's
+                # detected one; stored as -1 ("detected") so it transfers between
+                # matrices
+                if cfg["row_stride"] > 1:
+                    cfg["row_stride"] = -1
+                rows.append({"matrix": name, "nv": nv, **f, "nnz": a.nnz, **cfg, "ms": ms})
             best = min(ok, key=lambda x: x[1])
             print(f"{name} nv={nv}: {len(ok)} candidates, best {best[1] * 1e3:.1f} us "
                   f"{best[0].as_dict()}", flush=True)
@@ -130,7 +135,7 @@ def load(path):
         for r in csv.DictReader(fh):
             key = (r["matrix"], int(r["nv"]))
             data[key]["f"] = {k: float(r[k]) for k in FEATURES}
-            cfg = tuple(int(r[k]) for k in CFG_FIELDS)
+            cfg = tuple(int(r.get(k) or 0) for k in CFG_FIELDS)
             data[key]["t"][cfg] = float(r["ms"])
     return dict(data)
 
@@ -142,13 +147,13 @@ def heuristic_cfg(nv, table):
     lanes clamped to [16, 256] and unroll 8 -- looked up in the table by
     its defaults."""
     if nv == 1:
-        return (6, 0, 0, 1024, 0)
+        return (6, 0, 0, 1024, 0, 0)
     cands = [c for c in table if c[0] == 1 and c[4] == 8]
     if not cands:
         return None
     vec = max(c[1] for c in cands)
     lanes = min(64, 1 << max(0, math.ceil(math.log2(max(1, -(-nv // vec))))))
-    return (1, vec, lanes, max(16, min(256, 16 * lanes)), 8)
+    return (1, vec, lanes, max(16, min(256, 16 * lanes)), 8, 0)
 
 
 def scales(data):
@@ -233,12 +238,12 @@ def write_table(entries, sd, path, summary):
              "struct SelEntry {",
              "    int nv;",
              "    double f[5];                    // " + ", ".join(FEATURES),
-             "    int cfg[kSelTop][5];            // variant, vec, lanes, tile_nnz, unroll",
+             "    int cfg[kSelTop][6];            // variant, vec, lanes, tile_nnz, unroll, row_stride",
              "    float logratio[kSelTop];        // log(time / best time); 99 = no entry",
              "};",
              "static const SelEntry kSelTable[] = {"]
     for m, nv, f, top in sorted(entries, key=lambda e: (e[1], e[0])):
-        cfgs = [list(c) for c, _ in top] + [[0, 0, 0, 0, 0]] * (TOP - len(top))
+        cfgs = [list(c) for c, _ in top] + [[0] * len(CFG_FIELDS)] * (TOP - len(top))
         lrs = [lr for _, lr in top] + [99.0] * (TOP - len(top))
         lines.append(f"    {{{nv}, {{{', '.join(f'{f[k]:.6g}' for k in FEATURES)}}},  // {m}")
         lines.append("     {" + ", ".join("{" + ", ".join(str(x) for x in c) + "}" for c in cfgs)

@@ -276,10 +276,12 @@ typedef struct dspmm_config {
     int tile_nnz;   /* NNZ_TILED: max nonzeros per tile; 0 = auto */
     int unroll;     /* NNZ_TILED: gathers in flight per lane group; 0 = auto */
     int row_stride; /* ROW_BLOCKED: distance between the rows of a block, in
-                       [0, max(1, nr)]; 0 or 1 = consecutive rows.  Other
-                       variants ignore it.  The autotuner also tries the
-                       distance at which sampled rows share the most columns
-                       (a grid stencil's line or plane length). */
+                       [1, max(1, nr)]; 0 = consecutive rows; -1 = the
+                       distance at which the matrix's sampled rows share the
+                       most columns (a grid stencil's plane length; 1 when
+                       no distance stands out).  Other variants ignore it.
+                       The autotuner tries consecutive rows and that
+                       distance. */
 } dspmm_config;
 
 /* C = A * B, overwriting C (nv columns; ldb/ldc in elements, >= nv for

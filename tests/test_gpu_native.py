@@ -352,13 +352,21 @@ def test_row_blocked_strided(name, strides, R):
 
 
 def test_row_blocked_stride_bounds_and_autotune():
-    """row_stride outside [0, nr] is rejected; on a 20^3 stencil the
-    autotuner tries blocks of consecutive rows and blocks one plane (400
-    rows) apart: sampled rows share 16.8 columns on average with the row a
-    plane below, 16.0 with the row a line below, 11 or fewer at the other
-    candidate distances."""
+    """row_stride outside [-1, nr] is rejected; on a 20^3 stencil the
+    detected distance (row_stride -1) is one plane, 400 rows: sampled rows
+    share 16.8 columns on average with the row a plane below, 16.0 with the
+    row a line below, 11 or fewer at the other candidate distances.  The
+    autotuner tries blocks of consecutive rows and blocks 400 rows apart."""
     c, m = matrix("stencil_20")
-    for bad in (-1, c.nr + 1):
+    nv = 8
+    h32, _, _ = c.ref(RM, nv)
+    detected = run_native(c, m, nv, config=D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4,
+                                                         row_stride=-1))
+    assert_bitwise(detected, h32, "stencil_20 row-blocked detected stride")
+    p400 = run_native(c, m, nv, config=D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4,
+                                                     row_stride=400))
+    assert np.array_equal(bits(detected), bits(p400))
+    for bad in (-2, c.nr + 1):
         with pytest.raises(D.DspmmError) as e:
             m.spmm(c.d_B, c.d_C, 8, config=D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4,
                                                          row_stride=bad))

@@ -45,7 +45,7 @@ def test_table_compiles():
 
 def test_vote_prefers_fast_configuration_of_neighbours():
     f = {k: 0.0 for k in S.FEATURES}
-    a, b = (1, 4, 8, 32, 8), (2, 0, 0, 0, 0)
+    a, b = (1, 4, 8, 32, 8, 0), (2, 0, 0, 0, 0, 0)
     near = [("m1", 32, dict(f), [(a, 0.0), (b, 0.5)]),
             ("m2", 32, dict(f), [(a, 0.1), (b, 0.0)]),
             ("m3", 32, dict(f), [(a, 0.0), (b, 0.9)])]
@@ -58,7 +58,7 @@ def test_vote_prefers_fast_configuration_of_neighbours():
 
 
 def test_heuristic_config_matches_library_defaults():
-    table = {(1, 4, 8, 32, 8): 1.0, (1, 2, 16, 256, 8): 2.0, (1, 4, 8, 32, 4): 1.0}
-    assert S.heuristic_cfg(1, table) == (6, 0, 0, 1024, 0)
+    table = {(1, 4, 8, 32, 8, 0): 1.0, (1, 2, 16, 256, 8, 0): 2.0, (1, 4, 8, 32, 4, 0): 1.0}
+    assert S.heuristic_cfg(1, table) == (6, 0, 0, 1024, 0, 0)
     # nv = 32 with vec 4 -> 8 lanes -> tile 16 * 8 = 128, unroll 8
-    assert S.heuristic_cfg(32, table) == (1, 4, 8, 128, 8)
+    assert S.heuristic_cfg(32, table) == (1, 4, 8, 128, 8, 0)
