@@ -767,6 +767,7 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         if (cfg.variant == DSPMM_VARIANT_CSR_SCALAR && layout == DENSE_ROW_MAJOR &&
             (bf16 || cfg.unroll > 0)) {
             dspmm_config c = cfg;
+            if (cfg.tile_nnz != 0 && cfg.tile_nnz != 1) return DSPMM_STATUS_NOT_SUPPORTED;
             c.tile_nnz = 0;
             c.unroll = cfg.unroll > 0 ? cfg.unroll : 4;
             if (!row_seq_unroll_ok(c.unroll)) return DSPMM_STATUS_NOT_SUPPORTED;
@@ -787,6 +788,7 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
             a.slab_w = a.lanes * c.vec;
             a.nslabs = div_up(nv, a.slab_w);
             a.unroll = c.unroll;
+            a.sliced = cfg.tile_nnz == 1;
             a.bf16 = bf16;
             dspmm_status_t s = launch_row_seq(a, st);
             if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
@@ -1199,7 +1201,9 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
         }
     }
     {
-        // row-sequential (LDS-staged indices), over the gathers in flight
+        // row-sequential (LDS-staged indices), over the gathers in flight;
+        // sliced staging windows where some wavefront's rows overflow one
+        // window (otherwise both stagings run the same code)
         for (int u : {2, 4, 8, 16})
             for (int lanes : slab_lanes) {
                 if (lanes && (u == 2 || u == 16)) continue;
@@ -1207,6 +1211,22 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                 c.variant = DSPMM_VARIANT_CSR_SCALAR;
                 c.unroll = u;
                 c.lanes = lanes;
+                cand.push_back(c);
+            }
+        dspmm_config base{};
+        base.variant = DSPMM_VARIANT_NNZ_TILED;
+        complete_nb(base, nv, ldb, ldc, B, C, bf16);
+        const int g = 64 / std::min(base.lanes, make_slab(nv, base.vec).lanes);   // rows per wavefront
+        bool overflow = false;
+        for (int rw = 0; rw < A->nr && !overflow; rw += g)
+            overflow = A->h_rowPtr[(size_t)std::min(A->nr, rw + g)] - A->h_rowPtr[(size_t)rw] >
+                       kRowSeqWindow;
+        if (overflow)
+            for (int u : {4, 8, 16}) {
+                dspmm_config c{};
+                c.variant = DSPMM_VARIANT_CSR_SCALAR;
+                c.unroll = u;
+                c.tile_nnz = 1;
                 cand.push_back(c);
             }
     }

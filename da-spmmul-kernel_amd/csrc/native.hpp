@@ -88,9 +88,12 @@ struct RowSeqLaunch {
     int64_t ldb = 0, ldc = 0;
     int nr = 0, nv = 0;
     int vw = 1, lanes = 1, slab_w = 1, nslabs = 1, unroll = 4;
+    bool sliced = false;               // per-row slices of the staging windows
     bool bf16 = false;
 };
 bool row_seq_unroll_ok(int u);
+// Pairs a row-sequential wavefront stages per window.
+constexpr int kRowSeqWindow = 512;
 dspmm_status_t launch_row_seq(const RowSeqLaunch &a, hipStream_t st);
 
 // Arguments of one single-matrix nnz-tiled launch (plus its split combine).

@@ -972,22 +972,26 @@ dspmm_status_t launch_row_blocked(const RowBlockLaunch &a, hipStream_t st) {
 // Row-sequential SpMM (CSR_SCALAR with an unroll, and its BF16-B form).  A
 // group of 2^lg lanes owns one row, lane j of the group VW consecutive
 // columns of the slab; a wavefront owns G = 64 >> lg consecutive rows.  The
-// wavefront stages its rows' (col, val) into its own LDS region in windows
-// of kRsWin nonzeros with coalesced loads; each group then reads its row's
-// pairs from LDS as broadcasts, issues UNR gathers of B and adds the
-// products in storage order, so results equal the host loop's row sums.
+// wavefront stages its rows' (col, val) into its own LDS region: in one
+// coalesced pass when they fit kRsWin pairs, else in windows split into one
+// slice per group that still has pairs (so no group waits while another
+// group's row is staged, and a long row gets the whole window once the
+// others are done).  Each group then reads its row's pairs from
+// LDS as broadcasts, issues UNR gathers of B and adds the products in
+// storage order, so results equal the host loop's row sums.
 // The last chunk of a window is padded with its last entry (loads without
 // a branch keep the gathers back to back) and the padded products are
 // skipped.  No workgroup barriers; every row is overwritten (empty rows
 // -> 0).
 // ===========================================================================
-constexpr int kRsWin = 512;
+constexpr int kRsWin = kRowSeqWindow;
 
-template <typename TB, int VW, int UNR>
+template <typename TB, int VW, int UNR, bool SLICED>
 __global__ __launch_bounds__(kBlock) void k_row_seq(RowSeqLaunch g, int lg) {
     constexpr int WPB = kBlock / kWave;
     __shared__ int s_col[WPB][kRsWin];
     __shared__ float s_val[WPB][kRsWin];
+    __shared__ int s_tab[WPB][SLICED ? 2 * kWave : 1];   // per window: slice start, count per rank
     const int t = threadIdx.x, wv = t / kWave, lane = t % kWave;
     const int G = kWave >> lg;
     const int64_t rw = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * WPB + wv) * G;
@@ -1004,21 +1008,11 @@ __global__ __launch_bounds__(kBlock) void k_row_seq(RowSeqLaunch g, int lg) {
         end = g.rowPtr[r + 1];
     }
     const TB *__restrict__ B = reinterpret_cast<const TB *>(g.B) + col0;
-    int *sc = s_col[wv];
-    float *sv = s_val[wv];
     float acc[VW];
 #pragma unroll
     for (int k = 0; k < VW; k++) acc[k] = 0.f;
-    for (int w = e0; w < e1; w += kRsWin) {
-        const int cnt = min(kRsWin, e1 - w);
-        wave_lds_sync();
-        for (int i = lane; i < cnt; i += kWave) {
-            sc[i] = ld_stream(g.colIdx + w + i);
-            sv[i] = ld_stream(g.values + w + i);
-        }
-        wave_lds_sync();
-        if (!active) continue;
-        const int p0 = max(start, w) - w, p1 = min(end, w + cnt) - w;
+    // Adds products p0 .. p1-1 of the staged pairs, UNR gathers in flight.
+    auto run = [&](const int *sc, const float *sv, int p0, int p1) {
         for (int p = p0; p < p1; p += UNR) {
             float bv[UNR][VW];
 #pragma unroll
@@ -1033,24 +1027,83 @@ __global__ __launch_bounds__(kBlock) void k_row_seq(RowSeqLaunch g, int lg) {
                 }
             }
         }
+    };
+    if (!SLICED || e1 - e0 <= kRsWin) {
+        // windows of the wavefront's contiguous pairs, one coalesced staging
+        // pass each (a single window when they fit)
+        int *sc = s_col[wv];
+        float *sv = s_val[wv];
+        for (int w = e0; w < e1; w += kRsWin) {
+            const int cnt = min(kRsWin, e1 - w);
+            wave_lds_sync();
+            for (int i = lane; i < cnt; i += kWave) {
+                sc[i] = ld_stream(g.colIdx + w + i);
+                sv[i] = ld_stream(g.values + w + i);
+            }
+            wave_lds_sync();
+            if (active) run(sc, sv, max(start, w) - w, min(end, w + cnt) - w);
+        }
+    } else if constexpr (SLICED) {
+        // Windows shared by the groups that still have pairs: with n such
+        // groups each gets a slice of sl = kRsWin / n (rounded down to a
+        // power of two) pairs, so every group works in every window and a
+        // long row gets the whole window once the short ones are done.  The
+        // wavefront stages all slices together (lane i takes pair i of the
+        // concatenated slices), coalesced within each slice.
+        int *tab_s = s_tab[wv], *tab_c = s_tab[wv] + kWave;
+        const bool leader = (lane & ((1 << lg) - 1)) == 0;
+        const uint64_t below = (1ull << (gi << lg)) - 1;   // lanes before this group's leader
+        const int len = end - start;
+        int done = 0;
+        for (;;) {
+            const int rem = len - done;
+            const uint64_t m = __ballot(leader && rem > 0);
+            if (m == 0) break;
+            const int nact = __popcll(m);
+            int lsl = 9;                                    // log2 kRsWin
+            while ((nact << lsl) > kRsWin) lsl--;
+            const int sl = 1 << lsl;
+            const int rank = __popcll(m & below);
+            const int cnt = rem > 0 ? min(sl, rem) : 0;
+            wave_lds_sync();
+            if (leader && rem > 0) {
+                tab_s[rank] = start + done;
+                tab_c[rank] = cnt;
+            }
+            wave_lds_sync();
+            for (int i = lane; i < (nact << lsl); i += kWave) {
+                const int q = i >> lsl, o = i & (sl - 1);
+                if (o < tab_c[q]) {
+                    s_col[wv][i] = ld_stream(g.colIdx + tab_s[q] + o);
+                    s_val[wv][i] = ld_stream(g.values + tab_s[q] + o);
+                }
+            }
+            wave_lds_sync();
+            if (active && cnt > 0) run(s_col[wv] + (rank << lsl), s_val[wv] + (rank << lsl), 0, cnt);
+            done += cnt;
+        }
     }
     if (active) store_vec<VW>(g.C + r * g.ldc + col0, acc);
 }
 
 namespace {
-template <typename TB, int VW>
-dspmm_status_t launch_row_seq_v(const RowSeqLaunch &a, hipStream_t st) {
+template <typename TB, int VW, bool SL>
+dspmm_status_t launch_row_seq_vs(const RowSeqLaunch &a, hipStream_t st) {
     int lg = 0;
     while ((1 << lg) < a.lanes) lg++;
     const dim3 grid(div_up(a.nr, kBlock >> lg), a.nslabs);
     switch (a.unroll) {
-    case 2: hipLaunchKernelGGL((k_row_seq<TB, VW, 2>), grid, dim3(kBlock), 0, st, a, lg); break;
-    case 4: hipLaunchKernelGGL((k_row_seq<TB, VW, 4>), grid, dim3(kBlock), 0, st, a, lg); break;
-    case 8: hipLaunchKernelGGL((k_row_seq<TB, VW, 8>), grid, dim3(kBlock), 0, st, a, lg); break;
-    case 16: hipLaunchKernelGGL((k_row_seq<TB, VW, 16>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 2: hipLaunchKernelGGL((k_row_seq<TB, VW, 2, SL>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 4: hipLaunchKernelGGL((k_row_seq<TB, VW, 4, SL>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 8: hipLaunchKernelGGL((k_row_seq<TB, VW, 8, SL>), grid, dim3(kBlock), 0, st, a, lg); break;
+    case 16: hipLaunchKernelGGL((k_row_seq<TB, VW, 16, SL>), grid, dim3(kBlock), 0, st, a, lg); break;
     default: return DSPMM_STATUS_NOT_SUPPORTED;
     }
     return launch_status();
+}
+template <typename TB, int VW>
+dspmm_status_t launch_row_seq_v(const RowSeqLaunch &a, hipStream_t st) {
+    return a.sliced ? launch_row_seq_vs<TB, VW, true>(a, st) : launch_row_seq_vs<TB, VW, false>(a, st);
 }
 }  // namespace
 

@@ -243,7 +243,11 @@ typedef enum dspmm_variant {
                                     host loop's order); row-major with unroll
                                     2/4/8/16 or BF16 B: each wavefront stages
                                     its rows' (col, val) in LDS and keeps
-                                    `unroll` gathers in flight per group;
+                                    `unroll` gathers in flight per group
+                                    (tile_nnz 1: windows split into one slice
+                                    per unfinished row, for rows that overflow
+                                    a wavefront's 512-pair window; 0: windows
+                                    of the wavefront's contiguous pairs);
                                     unroll 0 with FP32 B: the reference-class
                                     kernel (workgroup-staged, 4 in flight) */
     DSPMM_VARIANT_CSR_VECTOR = 3,

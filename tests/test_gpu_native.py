@@ -409,6 +409,16 @@ def test_row_sequential(name, unroll):
         what = f"{name} row-seq unroll={unroll} nv={nv}"
         note(what)
         assert_bitwise(run_native(c, m, nv, config=cfg), h32, what)
+    # sliced staging windows (tile_nnz 1): wavefronts whose rows overflow one
+    # window give each unfinished row a slice of every window
+    sliced = D.make_config(D.VARIANT_CSR_SCALAR, unroll=unroll, tile_nnz=1)
+    for nv in (1, 3, 8, 64):
+        if nv > c.maxnv:
+            continue
+        h32, c64, ab = c.ref(RM, nv)
+        what = f"{name} row-seq sliced unroll={unroll} nv={nv}"
+        note(what)
+        assert_bitwise(run_native(c, m, nv, config=sliced), h32, what)
     nv = min(64, c.maxnv)
     h32, c64, ab = c.ref(RM, nv)
     narrow = D.make_config(D.VARIANT_CSR_SCALAR, vec=2, lanes=4, unroll=unroll)   # 8 slabs
@@ -416,7 +426,7 @@ def test_row_sequential(name, unroll):
     d16 = D.to_bf16_device(c.d_B)
     h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
                          O.round_bf16(c.dense(nv))).reshape(-1)
-    for bf in (cfg, D.make_config(D.VARIANT_CSR_SCALAR)):
+    for bf in (cfg, sliced, D.make_config(D.VARIANT_CSR_SCALAR)):
         out = run_native(c, m, nv, config=bf, btype=D.DSPMM_R_16BF, B=d16)
         assert_bitwise(out, h32r, f"{name} row-seq unroll={bf.unroll} bf16")
 
@@ -434,10 +444,11 @@ def test_row_sequential_leading_dimensions_and_bad_unroll():
     h32 = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
                         np.ascontiguousarray(B[:, :nv]))
     assert_bitwise(np.ascontiguousarray(out[:, :nv]).reshape(-1), h32.reshape(-1), "row-seq ld")
-    with pytest.raises(D.DspmmError) as e:
-        m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc,
-               config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=5))
-    assert e.value.status == D.STATUS_NOT_SUPPORTED
+    for bad in (D.make_config(D.VARIANT_CSR_SCALAR, unroll=5),
+                D.make_config(D.VARIANT_CSR_SCALAR, unroll=4, tile_nnz=2)):
+        with pytest.raises(D.DspmmError) as e:
+            m.spmm(d_B, d_C, nv, ldb=ldb, ldc=ldc, config=bad)
+        assert e.value.status == D.STATUS_NOT_SUPPORTED
 
 
 @pytest.mark.parametrize("name", ["c1", "zipf_mid", "edge", "stencil_20"])
