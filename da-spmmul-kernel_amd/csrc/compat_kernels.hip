@@ -71,8 +71,15 @@ __device__ __forceinline__ void stage_window(int *s_col, float *s_val, const int
 // owns VW consecutive columns of the slab.  The block's rows' nonzeros are
 // staged through LDS in windows; every lane reads its row's (col, val) as an
 // LDS broadcast within its group.  Overwrites every row (empty rows -> 0).
+// SLICED: a block whose rows overflow one window, and one of whose rows is
+// at least 4 shares of a window long (kWin / rows per block), splits each
+// window into one slice per row that still has nonzeros (kWin / n for n
+// such rows, rounded down to a power of two), staged by the whole block, so
+// that no row waits while windows cover only other rows; a long row gets
+// the whole window once the others are done.  Both forms add each row's
+// products in nonzero order.
 // ===========================================================================
-template <int VW, int L>
+template <int VW, int L, bool SLICED>
 __global__ __launch_bounds__(kBlock) void k_csr_scalar_rm(int nr, int nv, int64_t ldb, int64_t ldc,
                                                            const int *__restrict__ rowPtr,
                                                            const int *__restrict__ colIdx,
@@ -80,8 +87,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_scalar_rm(int nr, int nv, int64_
                                                            const float *__restrict__ B,
                                                            float *__restrict__ C, int slab_w) {
     constexpr int RPB = kBlock / L;
+    constexpr int NW = kBlock / kWave;
     __shared__ int s_col[kWin];
     __shared__ float s_val[kWin];
+    __shared__ int s_tab[SLICED ? 2 * RPB + NW : 1];   // slice start, count per rank; per-wave counts
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int r0 = b * RPB;
     const int r = r0 + (int)threadIdx.x / L;
@@ -94,13 +103,74 @@ __global__ __launch_bounds__(kBlock) void k_csr_scalar_rm(int nr, int nv, int64_
     float acc[VW];
 #pragma unroll
     for (int k = 0; k < VW; k++) acc[k] = 0.f;
-    for (int w = bs; w < be; w += kWin) {
-        const int cnt = min(kWin, be - w);
-        __syncthreads();
-        stage_window(s_col, s_val, colIdx, values, w, cnt);
-        __syncthreads();
-        const int p0 = max(start, w), p1 = min(end, w + cnt);
-        if (colok) seq_accumulate<float, VW, 4>(acc, p0, p1, w, s_col, s_val, B, ldb, col0);
+    bool slice = false;
+    if constexpr (SLICED) {
+        // slices pay where a row is long against its share of a window (kWin /
+        // RPB pairs): then windows of contiguous pairs cover few rows each and
+        // the others wait.  Rows shorter than 4 shares run in plain windows.
+        if (be - bs > kWin) {
+            int mx = end - start;
+#pragma unroll
+            for (int o = kWave / 2; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+            if ((int)threadIdx.x % kWave == 0) s_tab[2 * RPB + (int)threadIdx.x / kWave] = mx;
+            __syncthreads();
+            for (int k = 0; k < NW; k++) mx = max(mx, s_tab[2 * RPB + k]);
+            slice = mx >= 4 * (kWin / RPB);
+            __syncthreads();
+        }
+    }
+    if (!slice) {
+        for (int w = bs; w < be; w += kWin) {
+            const int cnt = min(kWin, be - w);
+            __syncthreads();
+            stage_window(s_col, s_val, colIdx, values, w, cnt);
+            __syncthreads();
+            const int p0 = max(start, w), p1 = min(end, w + cnt);
+            if (colok) seq_accumulate<float, VW, 4>(acc, p0, p1, w, s_col, s_val, B, ldb, col0);
+        }
+    } else if constexpr (SLICED) {
+        const int lane = (int)threadIdx.x % kWave, wv = (int)threadIdx.x / kWave;
+        const bool leader = (threadIdx.x % L) == 0;
+        const uint64_t below = (1ull << (lane & ~(L - 1))) - 1;   // lanes before this group's leader
+        int *tab_s = s_tab, *tab_c = s_tab + RPB, *wcnt = s_tab + 2 * RPB;
+        const int len = end - start;
+        int done = 0;
+        for (;;) {
+            const int rem = len - done;
+            const uint64_t m = __ballot(leader && rem > 0);
+            if (lane == 0) wcnt[wv] = __popcll(m);
+            __syncthreads();
+            int nact = 0, base = 0;
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                const int c = wcnt[k];
+                nact += c;
+                if (k < wv) base += c;
+            }
+            if (nact == 0) break;                         // the same in every thread
+            int lsl = 0;
+            while ((2 << lsl) * nact <= kWin) lsl++;      // largest power of two with nact * 2^lsl <= kWin
+            const int rank = base + __popcll(m & below);
+            const int cnt = rem > 0 ? min(1 << lsl, rem) : 0;
+            if (leader && rem > 0) {
+                tab_s[rank] = start + done;
+                tab_c[rank] = cnt;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < (nact << lsl); i += kBlock) {
+                const int q = i >> lsl, o = i & ((1 << lsl) - 1);
+                if (o < tab_c[q]) {
+                    s_col[i] = ld_stream(colIdx + tab_s[q] + o);
+                    s_val[i] = ld_stream(values + tab_s[q] + o);
+                }
+            }
+            __syncthreads();
+            if (colok && cnt > 0)
+                seq_accumulate<float, VW, 4>(acc, rank << lsl, (rank << lsl) + cnt, 0, s_col, s_val,
+                                             B, ldb, col0);
+            done += cnt;
+            __syncthreads();                              // before the next window's tables
+        }
     }
     if (r < nr && colok) store_vec<VW>(C + (int64_t)r * ldc + col0, acc);
 }
@@ -525,16 +595,24 @@ void with_vw(int vw, F &&f) {
     }
 }
 
-dspmm_status_t csr_scalar_rm(int nr, int nv, int64_t ldb, int64_t ldc, const int *rowPtr,
+dspmm_status_t csr_scalar_rm(int nr, int nnz, int nv, int64_t ldb, int64_t ldc, const int *rowPtr,
                              const int *colIdx, const float *values, const float *B, float *C,
                              hipStream_t st) {
     const Slab s = make_slab(nv, choose_vw(nv, ldb, ldc, B, C, 4, 4));
+    // sliced windows where a block's rows typically fill more than half a
+    // window (the mean row length decides; a block that fits one window
+    // runs the same code in both forms)
+    const bool sliced = (double)nnz / std::max(1, nr) * (kBlock / s.lanes) > kWin / 2;
     with_vw(s.vw, [&](auto vw_c) {
         with_lanes(s.lanes, [&](auto l_c) {
             constexpr int VW_ = decltype(vw_c)::value, L_ = decltype(l_c)::value;
             dim3 grid(div_up(nr, kBlock / L_), s.count);
-            hipLaunchKernelGGL((k_csr_scalar_rm<VW_, L_>), grid, dim3(kBlock), 0, st, nr, nv, ldb,
-                               ldc, rowPtr, colIdx, values, B, C, s.width);
+            if (sliced)
+                hipLaunchKernelGGL((k_csr_scalar_rm<VW_, L_, true>), grid, dim3(kBlock), 0, st, nr,
+                                   nv, ldb, ldc, rowPtr, colIdx, values, B, C, s.width);
+            else
+                hipLaunchKernelGGL((k_csr_scalar_rm<VW_, L_, false>), grid, dim3(kBlock), 0, st, nr,
+                                   nv, ldb, ldc, rowPtr, colIdx, values, B, C, s.width);
         });
     });
     return launch_status();
@@ -673,7 +751,7 @@ dspmm_status_t launch_compat(int which, int nr, int nc, int nnz, int nv, int64_t
     switch (which) {
     case K_SPMV_CSR_SCALAR:
     case K_SPMM_CSR_SCALAR_ROW:
-        return csr_scalar_rm(nr, nv, ldb, ldc, rowPtr, colIdx, values, B, C, st);
+        return csr_scalar_rm(nr, nnz, nv, ldb, ldc, rowPtr, colIdx, values, B, C, st);
     case K_SPMV_CSR_VECTOR:
     case K_SPMM_CSR_VECTOR_ROW:
         return csr_vector_rm(nr, nnz, nv, ldb, ldc, rowPtr, colIdx, values, B, C, st, hint);
