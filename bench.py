@@ -173,6 +173,31 @@ def bench_stencil(D, W, np, warmup, steps, n=100, nv=64):
     return res
 
 
+def bench_bf16_b(D, np, a, m, d_B, d_C, nv, warmup, steps):
+    """The headline matrix with B stored as BF16 (rounded to nearest even on
+    the device, FP32 accumulation): autotuned native kernel, GFLOP/s,
+    algorithmic GB/s with 2-byte B, and every output's error against an
+    FP64 product of the original FP32 B, relative to sum_p |a_p b_p|."""
+    import scipy.sparse as sp
+    import workloads as W
+    d16 = D.to_bf16_device(d_B)
+    best, _ = m.autotune(d16, d_C, nv, btype=D.DSPMM_R_16BF, reps=3)
+    mean, _, main = m.benchmark(d16, d_C, nv, btype=D.DSPMM_R_16BF, config=best, warmup=warmup,
+                                reps=steps)
+    m.spmm(d16, d_C, nv, btype=D.DSPMM_R_16BF, config=best)
+    D.synchronize()
+    out = d_C.numpy().reshape(-1)[:a.nr * nv].reshape(a.nr, nv).astype(np.float64)
+    B64 = d_B.numpy().reshape(-1)[:a.nc * nv].reshape(a.nc, nv).astype(np.float64)
+    A64 = sp.csr_matrix((a.values.astype(np.float64), a.colIdx, a.rowPtr), shape=(a.nr, a.nc))
+    rel = np.abs(out - A64 @ B64) / np.maximum(abs(A64) @ np.abs(B64), np.finfo(np.float64).tiny)
+    by = W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv, 2)
+    gbs = by / (float(np.mean(main)) * 1e-3) / 1e9
+    return {"config": best.as_dict(), "ms": mean,
+            "GFLOP/s": W.flops(a.nnz, nv) / (mean * 1e-3) / 1e9, "GB/s_algorithmic": gbs,
+            "roofline_frac": gbs / PEAK_HBM_GBS, "max_rel_err_vs_fp64": float(rel.max()),
+            "outputs_over_1e-2": int((rel > 1e-2).sum())}
+
+
 def bench_batched(D, W, np, count, reps=20):
     """The batch workload: `count` independent matrices (M=K ~ U{256..16384}, LogNormal
     mean row length, N ~ U{16..256}), each autotuned, then run (i) as one
@@ -377,6 +402,9 @@ def main():
                                         "GFLOP/s": fl / (t_cm * 1e-3) / 1e9,
                                         "best_reference_class_GFLOP/s": ref_cm}
         log(f"column-major N={nv}: {extra[f'column_major_N{nv}']}")
+        extra[f"bf16_B_N{nv}"] = bench_bf16_b(D, np, a, m, d_B, d_C, nv, args.warmup,
+                                              min(args.steps, 100))
+        log(f"bf16 B N={nv}: {extra[f'bf16_B_N{nv}']}")
         extra["rocsparse_c2"] = bench_rocsparse(D, a, d_rp, d_ri, d_ci, d_v, d_B, d_C,
                                                 sorted({1, 32, nv}))
         log(f"rocsparse: {extra['rocsparse_c2']}")
