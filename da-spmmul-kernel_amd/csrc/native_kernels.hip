@@ -281,16 +281,22 @@ __global__ __launch_bounds__(kBlock) void k_nnz_tiled_batched(const ItemDesc *__
 // Single-matrix and batched launches share this function, so their results
 // agree bit for bit.  A lane loads up to 16 chunk values before adding them,
 // so its loads overlap.
-__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp) {
+// Column blocks cb0, cb0 + cstep, ... of the row (cstep = 1: all of them).
+// A column's sum does not depend on which wavefront computes its block.
+__device__ __forceinline__ int combine_lanes(int nv) {
+    int Lw = 1;
+    while (Lw < nv && Lw < kWave) Lw <<= 1;
+    return Lw;
+}
+__device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp, int cb0, int cstep) {
     const int row = sp.y, slot0 = sp.z, nch = sp.w;
     const int ln = lane_id();
-    int Lw = 1;
-    while (Lw < d.nv && Lw < kWave) Lw <<= 1;
+    const int Lw = combine_lanes(d.nv);
     const int P = kWave / Lw;
     const int q = ln / Lw, jl = ln % Lw;
     const int per = (nch + P - 1) / P;
     const int c0 = min(nch, q * per), c1 = min(nch, c0 + per);
-    for (int jb = 0; jb < d.nv; jb += Lw) {          // wave-uniform loop
+    for (int jb = cb0 * Lw; jb < d.nv; jb += cstep * Lw) {   // wave-uniform loop
         const int j = jb + jl;
         float s = 0.f;
         if (j < d.nv) {
@@ -312,11 +318,22 @@ __device__ __forceinline__ void combine_split(const ItemDesc &d, int4 sp) {
     }
 }
 
+// One wavefront per (split row, column block of combine_lanes(nv) columns):
+// rows with nv > 64 spread their column blocks over wavefronts, which
+// shortens the longest rows' chains without changing any sum.
+// Workgroups of k_split_combine: one wavefront per (split row, column block).
+int combine_blocks(int nsplits, int nv) {
+    int lw = 1;
+    while (lw < nv && lw < kWave) lw <<= 1;
+    return (int)div_up((int64_t)nsplits * div_up(nv, lw), (int64_t)(kBlock / kWave));
+}
+
 __global__ __launch_bounds__(kBlock) void k_split_combine(ItemDesc d, const int4 *__restrict__ splits,
                                                           int nsplits) {
-    const int w = blockIdx.x * (kBlock / kWave) + (int)threadIdx.x / kWave;
-    if (w >= nsplits) return;                       // whole wavefronts leave; no barrier follows
-    combine_split(d, splits[w]);
+    const int nblk = (d.nv + combine_lanes(d.nv) - 1) / combine_lanes(d.nv);
+    const int64_t w = (int64_t)blockIdx.x * (kBlock / kWave) + (int)threadIdx.x / kWave;
+    if (w >= (int64_t)nsplits * nblk) return;       // whole wavefronts leave; no barrier follows
+    combine_split(d, splits[w / nblk], (int)(w % nblk), nblk);
 }
 
 // Batched: wavefront w finds its item by binary search in the prefix sums
@@ -332,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_split_combine_batched(const ItemDesc
         if (prefix[mid] <= w) lo = mid; else hi = mid - 1;
     }
     const ItemDesc &d = items[lo];
-    combine_split(d, d.splits[w - prefix[lo]]);
+    combine_split(d, d.splits[w - prefix[lo]], 0, 1);
 }
 
 // ---- SpMV (nv == 1) -----------------------------------------------------------
@@ -808,8 +825,8 @@ dspmm_status_t launch_nnz_tiled(const NbLaunch &a, hipStream_t st) {
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine, dim3(div_up(a.nsplits, kBlock / kWave)), dim3(kBlock), 0,
-                           st, a.item, a.splits, a.nsplits);
+        hipLaunchKernelGGL(k_split_combine, dim3(combine_blocks(a.nsplits, a.item.nv)), dim3(kBlock),
+                           0, st, a.item, a.splits, a.nsplits);
         s = launch_status();
     }
     return s;
@@ -1139,8 +1156,8 @@ dspmm_status_t launch_spmv_tiled(const NbLaunch &a, hipStream_t st) {
     if (s != DSPMM_STATUS_SUCCESS) return s;
     if (a.mid_event) (void)hipEventRecord(a.mid_event, st);
     if (a.nsplits > 0) {
-        hipLaunchKernelGGL(k_split_combine, dim3(div_up(a.nsplits, kBlock / kWave)), dim3(kBlock), 0,
-                           st, a.item, a.splits, a.nsplits);
+        hipLaunchKernelGGL(k_split_combine, dim3(combine_blocks(a.nsplits, a.item.nv)), dim3(kBlock),
+                           0, st, a.item, a.splits, a.nsplits);
         s = launch_status();
     }
     return s;
