@@ -51,6 +51,14 @@ def make_items(count, seed=40, bf16=False, mmax=3000):
     v = rng.uniform(-1, 1, ci.size).astype(np.float32)
     for nv in ((1, 3, 64) if not bf16 else (8, 64)):
         items.append(Item(len(items), W.Csr(900, 3000, rp, ci, v), nv, seed + 77 + nv, bf16))
+    # a row of 20000 nonzeros at nv = 100: 79 chunks of 256 in two column
+    # blocks, which the single-matrix combine sums with a workgroup per block
+    # and the batched combine with one wavefront
+    lens = np.array([3, 0, 20000, 5, 70, 1])
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(30000, int(k), replace=False)) for k in lens if k]).astype(np.int32)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    items.append(Item(len(items), W.Csr(6, 30000, rp, ci, v), 100, seed + 99, bf16))
     return items
 
 
