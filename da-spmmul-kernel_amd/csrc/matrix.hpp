@@ -49,7 +49,9 @@ void build_cached_tiles(const std::vector<int> &rp, const std::vector<int> &ci, 
                         int umax, int smax, std::vector<int4> &tiles, std::vector<int> &ucols,
                         std::vector<uint16_t> &lidx, double &reuse);
 
-// Row-blocked plan: blocks of R consecutive rows; per block the ascending
+// Row-blocked plan: blocks of R rows `stride` apart (block b holds rows
+// (b / stride) * R * stride + b % stride + i * stride; blocks past the last
+// row are empty; false if the block count overflows an int); per block the ascending
 // distinct columns of its rows, each with the mask of rows that have it, and
 // per listed column R values (the rows' values, 0 where the mask bit is
 // clear).  False if a row is unsorted or repeats a column (the plan then

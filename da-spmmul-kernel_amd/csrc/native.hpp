@@ -58,7 +58,7 @@ size_t nnz_cached_lds_bytes(int umax, int smax, int nv, int vw, bool bf16);
 constexpr int kCachedRowsMax = 256;    // rows per cached tile
 
 // One launch of the row-blocked kernel (k_row_blocked): blocks of `rows`
-// consecutive rows; list[boff[b] .. boff[b+1]) = {column, row mask} of block
+// rows `stride` apart; list[boff[b] .. boff[b+1]) = {column, row mask} of block
 // b in ascending column order; vals[u * rows + i] = row i's value at listed
 // column u (0 where the mask bit is clear).
 struct RowBlockLaunch {
