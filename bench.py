@@ -240,6 +240,16 @@ def bench_batched(D, W, np, count, reps=20):
     res["speedup_graph_vs_plain"] = plain / res["graph"]["ms_mean"]
     res["speedup_persistent_vs_plain"] = plain / res["persistent"]["ms_mean"]
     b.close()
+    # the same pass with every B stored as BF16 (FP32 accumulation): half the
+    # gathered bytes
+    b16 = [D.to_bf16_device(d_B) for _, _, d_B, _ in items]
+    bb = D.Batch([(mat, nv, d16, d_C) for (mat, nv, _, d_C), d16 in zip(items, b16)],
+                 btype=D.DSPMM_R_16BF)
+    ms16, _ = bb.benchmark(D.BATCH_PERSISTENT, warmup=3, reps=reps)
+    res["persistent_bf16_B"] = {"ms_mean": ms16, "req_per_s": count / (ms16 * 1e-3),
+                                "GFLOP/s": flops_total / (ms16 * 1e-3) / 1e9,
+                                "speedup_vs_fp32_persistent": res["persistent"]["ms_mean"] / ms16}
+    bb.close()
     return res
 
 
