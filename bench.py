@@ -58,12 +58,16 @@ def traffic_from_profile(kernel_prefix, workload_tag):
 
 
 def profiled_config(workload_tag):
-    """The kernel configuration of the committed rocprofv3 PMC summary."""
+    """The kernel configuration of the committed rocprofv3 PMC summary, with
+    every field of the current configuration struct (absent fields take
+    their zero default)."""
     try:
         with open(os.path.join(ROOT, "profiles", f"pmc_{workload_tag}.json")) as f:
-            return json.load(f)["meta"]["config"]
+            cfg = json.load(f)["meta"]["config"]
     except (OSError, KeyError, ValueError, TypeError):
         return None
+    import dspmm as D
+    return D.config_from_dict(cfg).as_dict()
 
 
 def prefer_profiled(D, np, m, d_B, d_C, nv, best, profiled, rounds=3, reps=20):
