@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_scalar_rm(int nr, int nv, int64_
             __syncthreads();                              // before the next window's tables
         }
     }
-    if (r < nr && colok) store_vec<VW>(C + (int64_t)r * ldc + col0, acc);
+    if (r < nr && colok) store_vec_nt<VW>(C + (int64_t)r * ldc + col0, acc);
 }
 
 // ===========================================================================
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector_rm(int nr, int nv, int64_
     }
 #pragma unroll
     for (int k = 0; k < VW; k++) acc[k] = butterfly_sum<L, T>(acc[k]);
-    if (r < nr && colok && g == 0) store_vec<VW>(C + (int64_t)r * ldc + col0, acc);
+    if (r < nr && colok && g == 0) store_vec_nt<VW>(C + (int64_t)r * ldc + col0, acc);
 }
 
 // ===========================================================================

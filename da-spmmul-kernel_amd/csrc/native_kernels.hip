@@ -1100,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void k_row_seq(RowSeqLaunch g, int lg) {
             done += cnt;
         }
     }
-    if (active) store_vec<VW>(g.C + r * g.ldc + col0, acc);
+    if (active) store_vec_nt<VW>(g.C + r * g.ldc + col0, acc);
 }
 
 namespace {
