@@ -76,3 +76,20 @@ def test_counter_mode_derived_figures(tmp_path):
             "L1 misses in flight per CU": "30", "vector-memory reads per wave": "25"}
     for name, value in want.items():
         assert f"| {name} | {value} |" in text, name
+
+
+def test_committed_headline_profile_feeds_the_bench():
+    """bench.py reports roofline.traffic only when the measured configuration
+    equals the profiled one; the committed profile's configuration must
+    compare equal field by field to a configuration of the current struct
+    (a field added later takes its zero default), and its traffic must be
+    readable."""
+    sys.path.insert(0, ROOT)
+    import bench
+    import dspmm as D
+    prof = bench.profiled_config("c2_n128")
+    assert prof is not None
+    assert prof == D.config_from_dict(prof).as_dict()
+    assert set(prof) == set(D.make_config().as_dict())
+    traffic = bench.traffic_from_profile("k_nnz_tiled", "c2_n128")
+    assert isinstance(traffic, int) and traffic > 0
