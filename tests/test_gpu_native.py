@@ -459,7 +459,9 @@ def test_column_major_through_transposes(name):
     a caller workspace, and entries beyond the leading rows are untouched."""
     c, m = matrix(name)
     cfgs = [D.make_config(D.VARIANT_NNZ_TILED), D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
-            D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4)]
+            D.make_config(D.VARIANT_CSR_SCALAR, unroll=8, tile_nnz=1),
+            D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4),
+            D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4, unroll=4, row_stride=-1)]
     for nv in (1, 3, 8, 64, 100):
         if nv > c.maxnv:
             continue
