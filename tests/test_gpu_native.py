@@ -528,3 +528,38 @@ def test_native_variants_on_empty_and_one_entry_matrices():
     del d_B
     empty.close()
     one.close()
+
+
+@pytest.mark.parametrize("name", ["zipf_mid", "edge"])
+def test_wide_dense_operands(name):
+    """nv = 300 (several column slabs, five combine column blocks): the
+    nnz-tiled kernel within the FP32 gate, the row-exact kernels bit-exact
+    with the host loop, FP32 and BF16-stored B."""
+    c, m = matrix(name)
+    nv = 300
+    B = W.gen_dense(301, c.nc * nv).reshape(c.nc, nv)
+    d_B = D.DeviceArray.from_numpy(B)
+    d_C = D.DeviceArray.from_numpy(np.full((c.nr, nv), np.nan, np.float32))
+    h32 = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, B).reshape(-1)
+    c64, ab = O.spmm_fp64(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, B)
+    rl = np.repeat(c.rowlen, nv)
+    for cfg in (D.make_config(D.VARIANT_NNZ_TILED), D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=16),
+                D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
+                D.make_config(D.VARIANT_CSR_SCALAR, unroll=8, tile_nnz=1),
+                D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4),
+                D.make_config(D.VARIANT_CSR_SCALAR)):
+        what = f"{name} nv={nv} {cfg.as_dict()}"
+        note(what)
+        m.spmm(d_B, d_C, nv, config=cfg)
+        D.synchronize()
+        out = d_C.numpy().reshape(-1)
+        if cfg.variant == D.VARIANT_NNZ_TILED:
+            assert_fp32(out, c64.reshape(-1), ab.reshape(-1), rl, what, h32)
+        else:
+            assert_bitwise(out, h32, what)
+    d16 = D.to_bf16_device(d_B)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(B)).reshape(-1)
+    m.spmm(d16, d_C, nv, btype=D.DSPMM_R_16BF, config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=4))
+    D.synchronize()
+    assert_bitwise(d_C.numpy().reshape(-1), h32r, f"{name} nv={nv} bf16 row-seq")
