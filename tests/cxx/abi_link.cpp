@@ -29,5 +29,12 @@ int main() {
     const float first = (float)(std::rand() % 1000 - 500) / 1000;
     if (f.get_host_array()[0] != first) return 5;
     std::printf("ok %s\n", dspmm_version());
+    // sizes of the public structs, compared with the Python mirror's
+    std::printf("sizes device_info=%zu matrix_desc=%zu config=%zu batch_item=%zu batch_info=%zu "
+                "server_config=%zu request=%zu completion=%zu coo=%zu server_stats=%zu\n",
+                sizeof(dspmm_device_info), sizeof(dspmm_matrix_desc), sizeof(dspmm_config),
+                sizeof(dspmm_batch_item), sizeof(dspmm_batch_info), sizeof(dspmm_server_config),
+                sizeof(dspmm_request), sizeof(dspmm_completion), sizeof(dspmm_coo),
+                sizeof(dspmm_server_stats));
     return 0;
 }

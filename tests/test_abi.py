@@ -161,3 +161,13 @@ def test_cxx_caller_compiles_links_and_runs(tmp_path):
     assert out.returncode == 0, out.stderr
     assert out.stdout.startswith("ok dspmm")
     assert out.stderr.count("nv must be power of 2") == 2
+    # the ctypes mirror lays out every public struct as the C header does
+    line = next(l for l in out.stdout.splitlines() if l.startswith("sizes "))
+    sizes = dict(kv.split("=") for kv in line.split()[1:])
+    import ctypes
+    mirror = {"device_info": D.DeviceInfo, "matrix_desc": D.MatrixDesc, "config": D.Config,
+              "batch_item": D.BatchItem, "batch_info": D.BatchInfo,
+              "server_config": D.ServerConfig, "request": D.Request, "completion": D.Completion,
+              "coo": D.Coo, "server_stats": D.ServerStats}
+    for name, cls in mirror.items():
+        assert int(sizes[name]) == ctypes.sizeof(cls), (name, sizes[name], ctypes.sizeof(cls))
