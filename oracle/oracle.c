@@ -1,6 +1,6 @@
 /*
  * oracle.c -- CPU restatement of the reference's host path (test
- * infrastructure only; see oracle.h).  Built with -O2 -ffp-contract=off so
+ * infrastructure only; see oracle.h).  Built with -O3 -ffp-contract=off so
  * that `acc += a*b` is a rounded multiply followed by a rounded add, like the
  * reference's host loop compiled for baseline x86-64 (Makefile:3 passes no
  * host arch flags, and baseline x86-64 has no FMA).

@@ -80,16 +80,18 @@ def test_counter_mode_derived_figures(tmp_path):
 
 def test_committed_headline_profile_feeds_the_bench():
     """bench.py reports roofline.traffic only when the measured configuration
-    equals the profiled one; the committed profile's configuration must
-    compare equal field by field to a configuration of the current struct
-    (a field added later takes its zero default), and its traffic must be
-    readable."""
+    equals the profiled one; the committed headline profile's configuration
+    must compare equal field by field to a configuration of the current
+    struct (a field added later takes its zero default), and its traffic
+    must be readable for exactly that configuration."""
     sys.path.insert(0, ROOT)
     import bench
     import dspmm as D
-    prof = bench.profiled_config("c2_n128")
+    prof = bench.profiled_config(bench.HEADLINE_TAG)
     assert prof is not None
     assert prof == D.config_from_dict(prof).as_dict()
     assert set(prof) == set(D.make_config().as_dict())
-    traffic = bench.traffic_from_profile("k_nnz_tiled", "c2_n128")
+    traffic = bench.traffic_from_profile(bench.HEADLINE_TAG, prof)
     assert isinstance(traffic, int) and traffic > 0
+    other = dict(prof, unroll=prof["unroll"] + 1)
+    assert bench.traffic_from_profile(bench.HEADLINE_TAG, other) is None
