@@ -5,13 +5,17 @@ Accuracy gates (the tolerance the tests apply):
   * bit-exact: row-balanced sequential kernels (SR_RB, spmv_csr_scalar,
     spmm_csr_scalar_row/col) and the native nnz-tiled kernel on rows that
     fit one tile equal the reference host loop bit for bit;
-  * FP32: |c - c64| <= max(1e-5, gamma_{n+1}) * sum_p |a_p b_p| per output,
-    where n is the row length and gamma_k = k u / (1 - k u), u = 2^-24.
-    This is the required 1e-5 relative error measured against the
-    output's condition scale (|c64| alone is meaningless under
-    cancellation); for rows longer than ~160 nonzeros 1e-5 is below the
-    worst-case error of ANY FP32 summation order, including the
-    reference's own host loop, so the rigorous bound applies there;
+  * FP32: every output within 1e-5 * max(|c64|, sum_p |a_p b_p|) of the
+    FP64 result (the required 1e-5 relative error, measured against the
+    output's condition scale: |c64| alone is meaningless under
+    cancellation).  Where the reference host loop itself misses that bar
+    (long rows: past ~160 nonzeros 1e-5 is below the worst case of any
+    FP32 summation order), the kernel may miss it on no more outputs than
+    the host loop does, and every output stays within the rigorous bound
+    gamma_{n+1} * sum_p |a_p b_p| (n the row length, gamma_k = k u / (1 - k u),
+    u = 2^-24).  Paths whose summation order differs from the host loop's
+    (split rows, tree reductions, atomics) are also pinned exactly by
+    integer-valued operands, where every order gives the same bits;
   * BF16-stored B: |c - c64| <= 1e-2 * sum_p |a_p b_p| against the FP64
     result on the original FP32 B;
   * integer-valued operands: exact, for every algorithm (atomics included).
@@ -99,26 +103,41 @@ def assert_bitwise(out, h32, what):
 def fp32_gate(out, c64, absum, rowlen):
     n = rowlen + 1.0
     gamma = n * U32 / (1.0 - n * U32)
-    tol = np.maximum(1e-5, gamma) * absum
+    tol = gamma * absum
     err = np.abs(out.astype(np.float64) - c64)
     return err, tol
 
 
-def assert_fp32(out, c64, absum, rowlen, what, h32=None):
+def assert_fp32(out, c64, absum, rowlen, what, h32):
+    """The FP32 gate: the count of outputs beyond 1e-5 relative error is at
+    most the host loop's own count on the same operands (h32), and every
+    output is within its row's rigorous summation bound."""
     assert np.all(np.isfinite(out)), f"{what}: non-finite outputs"
     err, tol = fp32_gate(out, c64, absum, rowlen)
     bad = err > tol
     if bad.any():
         i = int(np.flatnonzero(bad)[0])
-        raise AssertionError(f"{what}: {int(bad.sum())} outputs outside tolerance; first {i}: "
-                             f"err {err[i]:.3e} tol {tol[i]:.3e} (row len {rowlen[i]:.0f})")
-    # Report the literal 1e-5 relative error count beside the host loop's own.
+        raise AssertionError(f"{what}: {int(bad.sum())} outputs beyond the summation bound; "
+                             f"first {i}: err {err[i]:.3e} bound {tol[i]:.3e} "
+                             f"(row len {rowlen[i]:.0f})")
     rel = O.rel_error(out, c64, absum)
-    msg = f"{what}: max rel err {rel.max():.2e}, > 1e-5: {int((rel > 1e-5).sum())}"
-    if h32 is not None:
-        rel_h = O.rel_error(h32, c64, absum)
-        msg += f" (host loop: max {rel_h.max():.2e}, > 1e-5: {int((rel_h > 1e-5).sum())})"
-    note(msg)
+    rel_h = O.rel_error(h32, c64, absum)
+    n_gpu, n_host = int((rel > 1e-5).sum()), int((rel_h > 1e-5).sum())
+    note(f"{what}: max rel err {rel.max():.2e}, > 1e-5: {n_gpu} "
+         f"(host loop: max {rel_h.max():.2e}, > 1e-5: {n_host})")
+    assert n_gpu <= n_host, (f"{what}: {n_gpu} outputs beyond 1e-5 relative error, the host "
+                             f"loop has {n_host}")
+
+
+def assert_exact(out, c64, what):
+    """Integer-valued operands with every partial sum below 2^24: every
+    summation order gives the exact integer result."""
+    want = c64.astype(np.float32)
+    assert np.array_equal(want.astype(np.float64), c64), f"{what}: operands are not exact"
+    if not np.array_equal(bits(out), bits(want)):
+        bad = np.flatnonzero(bits(out) != bits(want))
+        raise AssertionError(f"{what}: {bad.size} outputs differ from the exact result; first at "
+                             f"{bad[0]}: {out[bad[0]]!r} vs {want[bad[0]]!r}")
 
 
 def assert_bf16(out, c64, absum, what):
@@ -184,6 +203,33 @@ def case(name):
         v = rng.integers(-8, 9, ci.size).astype(np.float32)
         B = rng.integers(-8, 9, 6000 * 64).astype(np.float32)
         c = Case(name, 5000, 6000, rp, ci, v, 64, B=B)
+    elif name == "int_zipf":
+        # power-law rows up to 3000 long (many split at every tile size),
+        # integer operands: |partial sums| <= 3000 * 64 < 2^24, so every
+        # summation order is exact
+        a = W.gen_csr(W.GEN_ZIPF, 20000, 20000, 31, 1.6, 1, 3000)
+        rng = np.random.default_rng(31)
+        v = rng.integers(-8, 9, a.nnz).astype(np.float32)
+        B = rng.integers(-8, 9, 20000 * 128).astype(np.float32)
+        c = Case(name, a.nr, a.nc, a.rowPtr, a.colIdx, v, 128, B=B)
+    elif name == "int_spmv":
+        # the SpMV plan's row classes at every tile size T = 512 .. 4096:
+        # rows of <= 16, 17..256 (shared tiles), 257..16 T (a tile of their
+        # own) and > 16 T nonzeros (carry chunks), integer operands
+        rng = np.random.default_rng(33)
+        lens = rng.integers(0, 17, 3000)
+        lens[100:400] = rng.integers(17, 257, 300)
+        lens[500:520] = rng.integers(257, 8193, 20)
+        lens[600] = 8192 + 5        # > 16 T for T = 512
+        lens[700] = 16384 + 77      # > 16 T for T <= 1024
+        lens[800] = 32768 + 300     # > 16 T for T <= 2048
+        lens[900] = 70000           # > 16 T for every T
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = np.concatenate([np.sort(rng.choice(100000, int(n), replace=False)) for n in lens
+                             if n]).astype(np.int32)
+        v = rng.integers(-8, 9, ci.size).astype(np.float32)
+        B = rng.integers(-8, 9, 100000 * 8).astype(np.float32)
+        c = Case(name, 3000, 100000, rp, ci, v, 8, B=B)
     else:
         raise KeyError(name)
     _CASES[name] = c

@@ -9,7 +9,7 @@ import pytest
 import dspmm as D
 import oracle as O
 import workloads as W
-from gpu_common import assert_bf16, assert_bitwise, assert_fp32, bits, note
+from gpu_common import assert_bf16, assert_bitwise, assert_exact, assert_fp32, bits, case, note
 
 pytestmark = pytest.mark.gpu
 RM = D.DENSE_ROW_MAJOR
@@ -32,6 +32,11 @@ class Item:
                               self.a.values, B)
         rl = np.repeat(np.diff(self.a.rowPtr).astype(np.float64), self.nv)
         return c64.reshape(-1), ab.reshape(-1), rl
+
+    def host32(self, B=None):
+        B = self.B if B is None else B
+        return O.spmm_host32(RM, self.a.nr, self.a.nc, self.nv, self.a.rowPtr, self.a.colIdx,
+                             self.a.values, B).reshape(-1)
 
 
 def make_items(count, seed=40, bf16=False, mmax=3000):
@@ -79,7 +84,7 @@ def test_batched_equals_single_and_oracle():
         out, ref = it.d_C.numpy().reshape(-1), it.d_C_ref.numpy().reshape(-1)
         assert_bitwise(out, ref, f"item {i} batched vs single")
         c64, ab, rl = it.oracle()
-        assert_fp32(out, c64, ab, rl, f"item {i} (M={it.a.nr}, nv={it.nv})")
+        assert_fp32(out, c64, ab, rl, f"item {i} (M={it.a.nr}, nv={it.nv})", it.host32())
 
 
 def test_plain_launches_match_batched():
@@ -116,7 +121,8 @@ def test_graph_replays_follow_rebound_operands():
         assert np.array_equal(bits(items[0].d_C_ref.numpy()), bits(items[0].d_C.numpy()))
         for i, it in enumerate(items):
             c64, ab, rl = it.oracle(news[i][0])
-            assert_fp32(it.d_C.numpy().reshape(-1), c64, ab, rl, f"replay {rep} item {i}")
+            assert_fp32(it.d_C.numpy().reshape(-1), c64, ab, rl, f"replay {rep} item {i}",
+                        it.host32(news[i][0]))
 
 
 def test_batched_bf16():
@@ -178,4 +184,50 @@ def test_batch_with_empty_and_tiny_matrices():
         else:
             c64, ab, rl = it.oracle()
             assert_fp32(out, c64, ab, rl, f"item {i}", h32)
+    b.close()
+
+
+def test_batched_split_rows_exact():
+    """The batched kernel and its split combine on integer operands (every
+    summation order exact): power-law rows up to 3000 long, a 4000-long row,
+    the SpMV ladder's rows up to 70000 long, and small matrices, in one
+    launch and as plain launches -- every output equals the exact product."""
+    items = []
+    for name, nv in (("int_zipf", 128), ("int_zipf", 32), ("int_valued", 64), ("int_spmv", 8),
+                     ("int_spmv", 1)):
+        c = case(name)
+        it = Item.__new__(Item)
+        it.a = W.Csr(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+        it.nv = nv
+        it.B = c.dense(nv).reshape(c.nc, nv)
+        it.m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+        it.d_B = D.DeviceArray.from_numpy(it.B)
+        it.d_C = D.DeviceArray.from_numpy(np.full((c.nr, nv), np.nan, np.float32))
+        items.append(it)
+    rng = np.random.default_rng(7)
+    for k in range(6):
+        mm = int(rng.integers(300, 3000))
+        a = W.gen_csr(W.GEN_BINOMIAL, mm, mm, 700 + k, p0=float(rng.integers(2, 60)))
+        vals = rng.integers(-8, 9, a.nnz).astype(np.float32)
+        it = Item.__new__(Item)
+        it.a = W.Csr(a.nr, a.nc, a.rowPtr, a.colIdx, vals)
+        it.nv = int(rng.choice([16, 32, 64, 128]))
+        it.B = rng.integers(-8, 9, (a.nc, it.nv)).astype(np.float32)
+        it.m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, vals)
+        it.d_B = D.DeviceArray.from_numpy(it.B)
+        it.d_C = D.DeviceArray.from_numpy(np.full((a.nr, it.nv), np.nan, np.float32))
+        items.append(it)
+    b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
+    info = b.describe()
+    note(f"integer batch: {info}")
+    assert info["splits"] > 0, "the batch must have split rows"
+    for mode, run in (("batched", b.run), ("graph", b.graph_launch), ("plain", b.run_plain)):
+        for it in items:
+            it.d_C.zero()
+        run()
+        D.synchronize()
+        for i, it in enumerate(items):
+            c64, _, _ = it.oracle()
+            assert_exact(it.d_C.numpy().reshape(-1), c64, f"{mode} item {i} (M={it.a.nr}, "
+                                                           f"nv={it.nv})")
     b.close()

@@ -150,7 +150,8 @@ def test_batch_of_512_equals_plain_launches():
         nv = items[i][1]
         c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
-        assert_fp32(batched[i], c64.reshape(-1), ab.reshape(-1), rl, f"c4 matrix {i}")
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        assert_fp32(batched[i], c64.reshape(-1), ab.reshape(-1), rl, f"c4 matrix {i}", h32)
     b.close()
     for it in items:
         it[0].close()

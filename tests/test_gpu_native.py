@@ -8,7 +8,7 @@ import pytest
 import dspmm as D
 import oracle as O
 import workloads as W
-from gpu_common import assert_bf16, assert_bitwise, assert_fp32, bits, case, note
+from gpu_common import assert_bf16, assert_bitwise, assert_exact, assert_fp32, bits, case, note
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +54,63 @@ def test_nnz_tiled_tile_shapes(name, S, unroll):
         # rows that fit a tile are summed exactly like the host loop
         assert_bitwise(out[~split], h32[~split], what + " (unsplit rows)")
         assert_fp32(out, c64, ab, c.rowlen_flat(RM, nv), what, h32)
+
+
+@pytest.mark.parametrize("name", ["int_valued", "int_zipf"])
+@pytest.mark.parametrize("S", [16, 64])
+def test_split_rows_exact(name, S):
+    """Rows longer than a tile are cut into chunks whose partial sums the
+    split combine adds in its own fixed order.  Integer operands make every
+    summation order exact, so every output -- split rows included -- must
+    equal the exact product bit for bit (a dropped, doubled or misplaced
+    chunk shows)."""
+    c, m = matrix(name)
+    assert (c.rowlen > S).sum() > 0, "the case must have split rows"
+    for nv in (1, 32, 64, 128):
+        if nv > c.maxnv:
+            continue
+        _, c64, _ = c.ref(RM, nv)
+        for vec, lanes in ((0, 0), (1, 64), (4, 8), (2, 16)):
+            cfg = D.make_config(D.VARIANT_NNZ_TILED, vec=vec, lanes=lanes, tile_nnz=S)
+            what = f"{name} split S={S} vec={vec} lanes={lanes} nv={nv}"
+            note(what)
+            try:
+                out = run_native(c, m, nv, config=cfg)
+            except D.DspmmError as e:
+                assert e.status == D.STATUS_NOT_SUPPORTED, what      # LDS budget
+                continue
+            assert_exact(out, c64, what)
+    # BF16-stored B: small integers are exact in BF16, so the same holds
+    nv = min(64, c.maxnv)
+    _, c64, _ = c.ref(RM, nv)
+    d16 = D.to_bf16_device(c.d_B)
+    out = run_native(c, m, nv, config=D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=S),
+                     btype=D.DSPMM_R_16BF, B=d16)
+    assert_exact(out, c64, f"{name} split S={S} bf16 nv={nv}")
+
+
+@pytest.mark.parametrize("T", [512, 1024, 2048, 4096])
+def test_spmv_tiled_exact(T):
+    """Every row class of the SpMV plan on integer operands: rows of <= 16
+    (one thread), 17..256 (the wavefront queue), 257..16 T (a tile of their
+    own, tree-reduced) and > 16 T nonzeros (carry chunks and the combine) --
+    all exact."""
+    c, m = matrix("int_spmv")
+    n = c.rowlen
+    for lo, hi in ((0, 16), (17, 256), (257, 16 * T), (16 * T + 1, 1 << 30)):
+        assert ((n >= lo) & (n <= hi)).any(), f"no row of {lo}..{hi} nonzeros"
+    _, c64, _ = c.ref(RM, 1)
+    for layout in (RM, CM):
+        what = f"int_spmv spmv T={T} layout={layout}"
+        note(what)
+        out = run_native(c, m, 1, layout=layout, config=D.make_config(D.VARIANT_SPMV_TILED,
+                                                                       tile_nnz=T))
+        assert_exact(out, c64, what)
+    for name in ("int_valued", "int_zipf"):
+        ci, mi = matrix(name)
+        _, c64, _ = ci.ref(RM, 1)
+        out = run_native(ci, mi, 1, config=D.make_config(D.VARIANT_SPMV_TILED, tile_nnz=T))
+        assert_exact(out, c64, f"{name} spmv T={T}")
 
 
 @pytest.mark.parametrize("vec", [1, 2, 4])
@@ -103,8 +160,9 @@ def test_leading_dimensions():
         assert np.all(out[:, nv:] == -3.0), "columns beyond nv must be left alone"
         dense = np.ascontiguousarray(B[:, :nv])
         c64, ab = O.spmm_fp64(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, dense)
+        h32 = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values, dense)
         assert_fp32(np.ascontiguousarray(out[:, :nv]).reshape(-1), c64.reshape(-1),
-                    ab.reshape(-1), np.repeat(c.rowlen, nv), "ld")
+                    ab.reshape(-1), np.repeat(c.rowlen, nv), "ld", h32.reshape(-1))
 
 
 @pytest.mark.parametrize("name", ["zipf_mid", "stencil_20", "edge"])

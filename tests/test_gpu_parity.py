@@ -108,7 +108,8 @@ def test_row_balanced_overwrites_and_nnz_balanced_accumulates():
     for _ in range(2):
         c.problem().run(D.DENSE_ROW_MAJOR, nv, D.ALG_COO_SCALAR)
     D.synchronize()
-    assert_fp32(c.output(nv) / 2, c64, ab, c.rowlen_flat(D.DENSE_ROW_MAJOR, nv), "2x accumulate")
+    assert_fp32(c.output(nv) / 2, c64, ab, c.rowlen_flat(D.DENSE_ROW_MAJOR, nv), "2x accumulate",
+                h32)
 
 
 @pytest.mark.parametrize("name", D.LAUNCHERS)

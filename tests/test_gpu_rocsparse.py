@@ -34,12 +34,12 @@ def test_comparator_columns_within_fp32_gate(name, nv):
             note(f"{what}: not supported by rocSPARSE")
             continue
         D.synchronize()
-        _, c64, ab = c.ref(layout, nv)
+        h32, c64, ab = c.ref(layout, nv)
         out = c.output(nv)
         if name == "int_valued":
             assert np.array_equal(out, c64.astype(np.float32)), what
         else:
-            assert_fp32(out, c64, ab, c.rowlen_flat(layout, nv), what)
+            assert_fp32(out, c64, ab, c.rowlen_flat(layout, nv), what, h32)
         ran += 1
     assert ran >= (3 if nv == 1 else 6), f"only {ran} comparator columns ran"
     rs.close()

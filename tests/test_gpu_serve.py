@@ -80,7 +80,8 @@ def test_server_alone_equals_in_the_mix():
         B = D.seeded_uniform(seed, a.nc * nv)
         c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
-        assert_fp32(alone[rid], c64.reshape(-1), ab.reshape(-1), rl, f"served request {rid}")
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        assert_fp32(alone[rid], c64.reshape(-1), ab.reshape(-1), rl, f"served request {rid}", h32)
     srv.close()
 
 
@@ -150,7 +151,8 @@ def test_plain_own_best_mode_is_correct():
         B = D.seeded_uniform(700 + k, a.nc * nv)
         c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
-        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"plain-own request {k}")
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"plain-own request {k}", h32)
     srv.close()
 
 
@@ -185,7 +187,8 @@ def test_large_requests_run_alone_with_their_own_kernel():
         B = D.seeded_uniform(seed, a.nc * nv)
         c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
-        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"mixed request {k}")
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        assert_fp32(got[k], c64.reshape(-1), ab.reshape(-1), rl, f"mixed request {k}", h32)
     srv.close()
 
 
@@ -214,8 +217,10 @@ def test_server_bf16_operands():
     for k, (a, nv) in enumerate(pool):
         B = D.seeded_uniform(500 + k, a.nc * nv)
         rl = np.repeat(np.diff(a.rowPtr).astype(np.float64), nv)
-        c64r, abr = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, O.round_bf16(B))
-        assert_fp32(alone[k], c64r.reshape(-1), abr.reshape(-1), rl, f"bf16 request {k}")
+        Br = O.round_bf16(B)
+        c64r, abr = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, Br)
+        h32r = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, Br).reshape(-1)
+        assert_fp32(alone[k], c64r.reshape(-1), abr.reshape(-1), rl, f"bf16 request {k}", h32r)
         c64, ab = O.spmm_fp64(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B)
         err = np.abs(alone[k].astype(np.float64) - c64.reshape(-1))
         assert np.all(err <= 1e-2 * ab.reshape(-1)), f"bf16 request {k}: beyond 1e-2"
