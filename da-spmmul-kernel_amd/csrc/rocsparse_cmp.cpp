@@ -1,5 +1,5 @@
 // rocsparse_cmp.cpp -- rocSPARSE generic SpMV/SpMM as the comparator columns
-// of the sweep (the role cuSPARSE has in cusparse_helper.h:136-470).
+// of the sweep (the role cuSPARSE has in cusparse_helper.h:120-437).
 // Built into lib/libdspmm_rocsparse.so; the kernel library does not link it.
 //
 // Per (nv, column) the handle keeps a prepared call: dense descriptors, the

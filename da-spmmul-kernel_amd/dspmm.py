@@ -53,10 +53,13 @@ VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
                  4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled", 7: "nnz_cached",
                  8: "row_blocked"}
 MATRIX_WITH_COO = 0x1
+PREPARE_HANDLE_BUFFERS = 0x1
 
 STATUS_SUCCESS = 0
 STATUS_INVALID_VALUE = 1
 STATUS_NOT_SUPPORTED = 2
+STATUS_WORKSPACE_TOO_SMALL = 5
+STATUS_NOT_PREPARED = 7
 
 # The 12 reference-named launchers, in spmv.h / spmm.h order.
 LAUNCHERS = ["spmv_csr_scalar", "spmv_coo_scalar", "spmv_csr_vector", "spmv_coo_vector",
@@ -187,6 +190,8 @@ def _declare(lib):
         "dspmm_matrix_describe": ([_P, C.POINTER(MatrixDesc)], _ST),
         "dspmm_spmm_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _P, _SZ, _P],
                           _ST),
+        "dspmm_matrix_prepare": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), C.c_uint],
+                                 _ST),
         "dspmm_spmm_ex_workspace_size": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(_SZ)],
                                          _ST),
         "dspmm_spmm_ex_resolve": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(Config)], _ST),
@@ -441,6 +446,17 @@ class Matrix:
                                  C.byref(config) if config is not None else None,
                                  ptr(workspace), workspace_bytes, stream)
         check(st, "spmm_ex")
+
+    def prepare(self, B, C_out, nv, layout=DENSE_ROW_MAJOR, ldb=None, ldc=None,
+                btype=DSPMM_R_32F, config: Config | None = None, handle_buffers=False):
+        """dspmm_matrix_prepare: builds the plans (and optionally sizes the
+        handle's buffers) a later spmm with these arguments needs."""
+        ldb = ldb if ldb is not None else (nv if layout == DENSE_ROW_MAJOR else self.nc)
+        ldc = ldc if ldc is not None else (nv if layout == DENSE_ROW_MAJOR else self.nr)
+        check(lib().dspmm_matrix_prepare(self.handle, layout, nv, ptr(B), ldb, btype, ptr(C_out),
+                                         ldc, C.byref(config) if config is not None else None,
+                                         PREPARE_HANDLE_BUFFERS if handle_buffers else 0),
+              "matrix_prepare")
 
     def resolve(self, nv, layout=DENSE_ROW_MAJOR, btype=DSPMM_R_32F, config=None) -> Config:
         out = Config()
@@ -721,7 +737,7 @@ def rocsparse_lib():
 class RocSparse:
     """rocSPARSE generic SpMV/SpMM over the caller's device arrays: the
     comparator columns of the sweep (CusparseSpmmProblem,
-    cusparse_helper.h:136-470).  Dense operands as the reference lays them
+    cusparse_helper.h:120-437).  Dense operands as the reference lays them
     out: row-major ld = nv, column-major ld = nc (B) / nr (C)."""
 
     def __init__(self, nr, nc, nnz, rowPtr, rowIdx, colIdx, values, dnInput, dnOutput,

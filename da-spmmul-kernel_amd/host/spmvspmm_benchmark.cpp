@@ -3,7 +3,7 @@
 // nv = 1, 2, 4, ..., maxNv with GFLOP/s of the 8 reference algorithms and
 // the rocSPARSE comparator columns.
 //
-// Follows spmvspmm_benchmark.cu:329-420 (main), :155-223 (checkAlgorithms),
+// Follows spmvspmm_benchmark.cu:329-419 (main), :155-223 (checkAlgorithms),
 // :237-327 (benchmarkAlgorithms), :79-90 (calcMaxNv) and run.sh (the CSV
 // header).  Usage, as the reference's binary:
 //

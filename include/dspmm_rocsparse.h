@@ -1,7 +1,7 @@
 /* dspmm_rocsparse.h -- the vendor-library comparator beside the hand-written
  * kernels: rocSPARSE's generic SpMV/SpMM on the same device arrays.
  *
- * Replaces CusparseSpmmProblem (cusparse_helper.h:136-470): its constructor
+ * Replaces CusparseSpmmProblem (cusparse_helper.h:120-437): its constructor
  * (:143-178), run(format, layout, nv, alg) (:215-231, :360-384) and
  * benchmark(...) (:233-271, :385-449), as driven by the sweep's
  * cusparse_* columns (spmvspmm_benchmark.cu:304-322, run.sh:8).  Built as its
