@@ -114,6 +114,8 @@ const char *dspmm_get_error_string(dspmm_status_t s) {
     case DSPMM_STATUS_HIP_ERROR: return "HIP runtime error";
     case DSPMM_STATUS_WORKSPACE_TOO_SMALL: return "workspace too small";
     case DSPMM_STATUS_ADMISSION_REJECTED: return "admission rejected (HBM budget)";
+    case DSPMM_STATUS_NOT_PREPARED:
+        return "not prepared (call dspmm_matrix_prepare with these arguments first)";
     }
     return "unknown status";
 }

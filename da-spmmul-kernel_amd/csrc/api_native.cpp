@@ -272,21 +272,12 @@ bool build_row_blocks(const std::vector<int> &rp, const std::vector<int> &ci,
 
 int cached_smax(int umax) { return std::min(4096, 4 * umax); }
 
-// Host copy of the column indices (device -> host, for plans that need them).
-dspmm_status_t host_colidx(const dspmm_matrix_s *A, std::vector<int> &ci) {
-    ci.resize((size_t)A->nnz);
-    if (A->nnz == 0) return DSPMM_STATUS_SUCCESS;
-    return hip_status(hipMemcpy(ci.data(), A->colIdx, sizeof(int) * (size_t)A->nnz,
-                                hipMemcpyDeviceToHost));
-}
-
 double column_reuse(dspmm_matrix_s *A) {
     if (A->reuse16 >= 0) return A->reuse16;
     A->reuse16 = 1.0;
     if (A->nnz == 0 || A->nr == 0) return A->reuse16;
     // up to 256 windows of 16 consecutive rows, spread over the matrix
     const int win = 16, nwin = std::min(256, std::max(1, A->nr / win));
-    std::vector<int> buf;
     std::vector<int> cols;
     int64_t nz = 0, distinct = 0;
     for (int w = 0; w < nwin; w++) {
@@ -294,11 +285,7 @@ double column_reuse(dspmm_matrix_s *A) {
         const int r1 = std::min(A->nr, r0 + win);
         const int p0 = A->h_rowPtr[(size_t)r0], p1 = A->h_rowPtr[(size_t)r1];
         if (p1 <= p0) continue;
-        buf.resize((size_t)(p1 - p0));
-        if (hipMemcpy(buf.data(), A->colIdx + p0, sizeof(int) * buf.size(), hipMemcpyDeviceToHost) !=
-            hipSuccess)
-            return A->reuse16;
-        cols.assign(buf.begin(), buf.end());
+        cols.assign(A->h_colIdx.begin() + p0, A->h_colIdx.begin() + p1);
         std::sort(cols.begin(), cols.end());
         nz += (int64_t)cols.size();
         distinct += (int64_t)(std::unique(cols.begin(), cols.end()) - cols.begin());
@@ -311,8 +298,7 @@ int shared_row_stride(dspmm_matrix_s *A) {
     if (A->rb_stride >= 1) return A->rb_stride;
     A->rb_stride = 1;
     if (A->nnz == 0 || A->nr < 3) return A->rb_stride;
-    std::vector<int> ci;
-    if (host_colidx(A, ci) != DSPMM_STATUS_SUCCESS) return A->rb_stride;
+    const std::vector<int> &ci = A->h_colIdx;
     const std::vector<int> &rp = A->h_rowPtr;
     const int nsamp = std::min(256, A->nr);
     std::vector<int> sample((size_t)nsamp);
@@ -361,10 +347,12 @@ int shared_row_stride(dspmm_matrix_s *A) {
     return A->rb_stride;
 }
 
-dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int stride) {
+dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int stride,
+                        bool build) {
     const int64_t key = 4 * ((int64_t)S + 16 * (int64_t)(stride - 1)) + kind;
     auto it = A->plans.find(key);
     if (it != A->plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
+    if (!build) return DSPMM_STATUS_NOT_PREPARED;
     std::vector<int4> tiles, splits;
     std::vector<int> ucols;
     std::vector<uint16_t> lidx;
@@ -373,23 +361,17 @@ dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int st
     if (kind == kPlanSpmv) {
         build_spmv_tiles(A->h_rowPtr, A->nr, S, kSpmvRowMax, tiles, splits, nslots);
     } else if (kind == kPlanCached) {
-        std::vector<int> ci;
-        dspmm_status_t s = host_colidx(A, ci);
-        if (s != DSPMM_STATUS_SUCCESS) return s;
-        build_cached_tiles(A->h_rowPtr, ci, A->nr, A->nc, S, cached_smax(S), tiles, ucols, lidx,
-                           reuse);
+        build_cached_tiles(A->h_rowPtr, A->h_colIdx, A->nr, A->nc, S, cached_smax(S), tiles, ucols,
+                           lidx, reuse);
     } else if (kind == kPlanRowBlock) {
-        std::vector<int> ci, boff;
-        std::vector<float> va((size_t)A->nnz), rvals;
+        std::vector<int> boff;
+        std::vector<float> rvals;
         std::vector<int2> list;
-        dspmm_status_t s = host_colidx(A, ci);
-        if (s == DSPMM_STATUS_SUCCESS && A->nnz)
-            s = hip_status(hipMemcpy(va.data(), A->values, sizeof(float) * (size_t)A->nnz,
-                                     hipMemcpyDeviceToHost));
-        if (s != DSPMM_STATUS_SUCCESS) return s;
+        dspmm_status_t s = DSPMM_STATUS_SUCCESS;
         NbPlan p;
         p.S = S;
-        p.usable = build_row_blocks(A->h_rowPtr, ci, va, A->nr, S, stride, boff, list, rvals);
+        p.usable = build_row_blocks(A->h_rowPtr, A->h_colIdx, A->h_values, A->nr, S, stride, boff,
+                                    list, rvals);
         if (p.usable) {
             p.ntiles = (int)boff.size() - 1;
             p.reuse = list.empty() ? 1.0 : (double)A->nnz / (double)list.size();
@@ -524,15 +506,58 @@ dspmm_status_t zero_dense(float *C, int rows, int cols, int64_t ld, hipStream_t 
     return launch_status();
 }
 
+// What a run_ex call may do besides launching.
+enum RunMode {
+    kRunLaunch = 0,     // launch; build missing plans and grow the handle's
+                        // buffers first, unless the stream is being captured
+    kRunPlans = 1,      // build the plans the call needs; launch nothing
+    kRunPrepare = 2,    // build its plans and size the handle's buffers; launch nothing
+    kRunNoAlloc = 3,    // launch with everything prepared; allocate and copy nothing
+};
+
+bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// The handle's own buffer (carry or transposed copies) of at least `need`
+// bytes, grown where the mode allows it.  Sets *buf = nullptr (and
+// succeeds) in kRunPlans mode when it is too small: nothing is launched.
+dspmm_status_t handle_buffer(void **buf, size_t *have, size_t need, RunMode mode, bool grow,
+                             void **out) {
+    *out = nullptr;
+    if (*have >= need) {
+        *out = *buf;
+        return DSPMM_STATUS_SUCCESS;
+    }
+    if (mode == kRunPlans) return DSPMM_STATUS_SUCCESS;
+    if (!grow) return DSPMM_STATUS_NOT_PREPARED;
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    dspmm_status_t s = hip_status(hipMalloc(buf, need));
+    if (s != DSPMM_STATUS_SUCCESS) return s;
+    *have = need;
+    *out = *buf;
+    return DSPMM_STATUS_SUCCESS;
+}
+
 // `mid` (may be null) is recorded on the stream after the main kernel of
 // the call (before the split-row fix-up of the nnz-tiled variant).
 dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void *B, int64_t ldb,
                       dspmm_dtype_t bt, float *C, int64_t ldc, const dspmm_config &cfg,
-                      void *ws, size_t ws_bytes, hipStream_t st, hipEvent_t mid = nullptr) {
+                      void *ws, size_t ws_bytes, hipStream_t st, hipEvent_t mid = nullptr,
+                      RunMode mode = kRunLaunch) {
     const bool bf16 = bt == DSPMM_R_16BF;
+    const bool launch = mode == kRunLaunch || mode == kRunNoAlloc;
+    // plans may be built and the handle's buffers grown here (never under
+    // stream capture, where a blocking copy or an allocation would break it)
+    const bool lazy = mode == kRunLaunch && !capturing(st);
+    const bool build = lazy || mode == kRunPlans || mode == kRunPrepare;
+    const bool grow = lazy || mode == kRunPrepare;
     if (layout == DENSE_COL_MAJOR && rm_native(cfg, bf16)) {
         if (nv == 1)      // x and y are contiguous: the row-major call with unit strides
-            return run_ex(A, DENSE_ROW_MAJOR, 1, B, 1, bt, C, 1, cfg, ws, ws_bytes, st, mid);
+            return run_ex(A, DENSE_ROW_MAJOR, 1, B, 1, bt, C, 1, cfg, ws, ws_bytes, st, mid, mode);
         const size_t tb = transposed_bytes(A, nv, bf16);
         char *base = nullptr;
         if (ws) {
@@ -540,28 +565,43 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
             if (!aligned(ws, 256)) return DSPMM_STATUS_INVALID_VALUE;
             base = reinterpret_cast<char *>(ws);
         } else {
-            if (A->tbuf_bytes < tb) {
-                hipFree(A->tbuf);
-                A->tbuf = nullptr;
-                A->tbuf_bytes = 0;
-                dspmm_status_t s = hip_status(hipMalloc(&A->tbuf, tb));
-                if (s != DSPMM_STATUS_SUCCESS) return s;
-                A->tbuf_bytes = tb;
-            }
-            base = reinterpret_cast<char *>(A->tbuf);
+            void *buf = nullptr;
+            dspmm_status_t s = handle_buffer(&A->tbuf, &A->tbuf_bytes, tb, mode, grow, &buf);
+            if (s != DSPMM_STATUS_SUCCESS) return s;
+            base = reinterpret_cast<char *>(buf);
         }
         const int64_t ld = transposed_ld(nv);
         const size_t bbytes = ((size_t)A->nc * (size_t)ld * (bf16 ? 2 : 4) + 255) / 256 * 256;
         void *Bt = base;
-        float *Ct = reinterpret_cast<float *>(base + bbytes);
+        float *Ct = base ? reinterpret_cast<float *>(base + bbytes) : nullptr;
+        dspmm_status_t s = DSPMM_STATUS_SUCCESS;
         // column-major B is nv rows of nc (stride ldb); B^T is nc rows of ld
-        dspmm_status_t s = launch_transpose(B, ldb, nv, A->nc, Bt, ld, bf16 ? 2 : 4, st);
+        if (launch) s = launch_transpose(B, ldb, nv, A->nc, Bt, ld, bf16 ? 2 : 4, st);
         if (s == DSPMM_STATUS_SUCCESS)
-            s = run_ex(A, DENSE_ROW_MAJOR, nv, Bt, ld, bt, Ct, ld, cfg,
-                       ws ? base + tb : nullptr, ws ? ws_bytes - tb : 0, st, mid);
-        if (s == DSPMM_STATUS_SUCCESS) s = launch_transpose(Ct, ld, A->nr, nv, C, ldc, 4, st);
+            s = run_ex(A, DENSE_ROW_MAJOR, nv, Bt ? Bt : reinterpret_cast<const void *>(256), ld,
+                       bt, Ct ? Ct : reinterpret_cast<float *>(256), ld, cfg,
+                       ws ? base + tb : nullptr, ws ? ws_bytes - tb : 0, st, mid, mode);
+        if (s == DSPMM_STATUS_SUCCESS && launch)
+            s = launch_transpose(Ct, ld, A->nr, nv, C, ldc, 4, st);
         return s;
     }
+    // Carry buffer of `need` bytes for split rows: the caller's workspace, or
+    // the handle's own buffer.
+    auto carry_buffer = [&](size_t need, int align, float **carry) -> dspmm_status_t {
+        *carry = nullptr;
+        if (need == 0) return DSPMM_STATUS_SUCCESS;
+        if (ws) {
+            if (ws_bytes < need) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
+            if (!aligned(ws, align)) return DSPMM_STATUS_INVALID_VALUE;
+            *carry = reinterpret_cast<float *>(ws);
+            return DSPMM_STATUS_SUCCESS;
+        }
+        void *buf = nullptr;
+        dspmm_status_t s = handle_buffer(reinterpret_cast<void **>(&A->carry), &A->carry_bytes,
+                                         need, mode, grow, &buf);
+        *carry = reinterpret_cast<float *>(buf);
+        return s;
+    };
     switch (cfg.variant) {
     case DSPMM_VARIANT_NNZ_TILED: {
         if (layout != DENSE_ROW_MAJOR) return DSPMM_STATUS_NOT_SUPPORTED;
@@ -569,28 +609,12 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         complete_nb(c, nv, ldb, ldc, B, C, bf16);
         if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) return DSPMM_STATUS_NOT_SUPPORTED;
         NbPlan *plan = nullptr;
-        dspmm_status_t s = get_plan(A, c.tile_nnz, &plan);
+        dspmm_status_t s = get_plan(A, c.tile_nnz, &plan, kPlanNnzTiled, 1, build);
         if (s != DSPMM_STATUS_SUCCESS) return s;
         const int64_t ldcar = carry_ld(nv);
-        const size_t need = sizeof(float) * (size_t)plan->nslots * (size_t)ldcar;
         float *carry = nullptr;
-        if (need > 0) {
-            if (ws) {
-                if (ws_bytes < need) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
-                if (!aligned(ws, 32)) return DSPMM_STATUS_INVALID_VALUE;
-                carry = reinterpret_cast<float *>(ws);
-            } else {
-                if (A->carry_bytes < need) {
-                    hipFree(A->carry);
-                    A->carry = nullptr;
-                    A->carry_bytes = 0;
-                    s = hip_status(hipMalloc((void **)&A->carry, need));
-                    if (s != DSPMM_STATUS_SUCCESS) return s;
-                    A->carry_bytes = need;
-                }
-                carry = A->carry;
-            }
-        }
+        s = carry_buffer(sizeof(float) * (size_t)plan->nslots * (size_t)ldcar, 32, &carry);
+        if (s != DSPMM_STATUS_SUCCESS || !launch) return s;
         NbLaunch a;
         ItemDesc &d = a.item;
         d.tiles = plan->d_tiles;
@@ -633,8 +657,8 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         if (nnz_cached_lds_bytes(umax, smax, nv, c.vec, bf16) > kMaxCachedLds)
             return DSPMM_STATUS_NOT_SUPPORTED;
         NbPlan *plan = nullptr;
-        dspmm_status_t s = get_plan(A, umax, &plan, kPlanCached);
-        if (s != DSPMM_STATUS_SUCCESS) return s;
+        dspmm_status_t s = get_plan(A, umax, &plan, kPlanCached, 1, build);
+        if (s != DSPMM_STATUS_SUCCESS || !launch) return s;
         CachedLaunch a;
         a.tiles = plan->d_tiles;
         a.ntiles = plan->ntiles;
@@ -677,11 +701,13 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         complete_nb(c, nv, ldb, ldc, B, C, bf16);
         if (cfg.row_stride < -1 || cfg.row_stride > std::max(1, A->nr))
             return DSPMM_STATUS_INVALID_VALUE;
+        if (cfg.row_stride < 0 && A->rb_stride < 1 && !build) return DSPMM_STATUS_NOT_PREPARED;
         const int stride = cfg.row_stride < 0 ? shared_row_stride(A) : std::max(1, cfg.row_stride);
         NbPlan *plan = nullptr;
-        dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock, stride);
+        dspmm_status_t s = get_plan(A, R, &plan, kPlanRowBlock, stride, build);
         if (s != DSPMM_STATUS_SUCCESS) return s;
         if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
+        if (!launch) return DSPMM_STATUS_SUCCESS;
         const Slab sl = make_slab(nv, c.vec);
         RowBlockLaunch a;
         a.boff = plan->d_boff;
@@ -712,27 +738,12 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         const int T = cfg.tile_nnz > 0 ? cfg.tile_nnz : kSpmvTile;
         if (!spmv_tile_ok(T)) return DSPMM_STATUS_NOT_SUPPORTED;
         NbPlan *plan = nullptr;
-        dspmm_status_t s = get_plan(A, T, &plan, kPlanSpmv);
+        dspmm_status_t s = get_plan(A, T, &plan, kPlanSpmv, 1, build);
         if (s != DSPMM_STATUS_SUCCESS) return s;
         const int64_t ldcar = carry_ld(1);
-        const size_t need = sizeof(float) * (size_t)plan->nslots * (size_t)ldcar;
         float *carry = nullptr;
-        if (need > 0) {
-            if (ws) {
-                if (ws_bytes < need) return DSPMM_STATUS_WORKSPACE_TOO_SMALL;
-                carry = reinterpret_cast<float *>(ws);
-            } else {
-                if (A->carry_bytes < need) {
-                    (void)hipFree(A->carry);
-                    A->carry = nullptr;
-                    A->carry_bytes = 0;
-                    s = hip_status(hipMalloc((void **)&A->carry, need));
-                    if (s != DSPMM_STATUS_SUCCESS) return s;
-                    A->carry_bytes = need;
-                }
-                carry = A->carry;
-            }
-        }
+        s = carry_buffer(sizeof(float) * (size_t)plan->nslots * (size_t)ldcar, 4, &carry);
+        if (s != DSPMM_STATUS_SUCCESS || !launch) return s;
         NbLaunch a;
         ItemDesc &d = a.item;
         d.tiles = plan->d_tiles;
@@ -771,6 +782,7 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
             c.tile_nnz = 0;
             c.unroll = cfg.unroll > 0 ? cfg.unroll : 4;
             if (!row_seq_unroll_ok(c.unroll)) return DSPMM_STATUS_NOT_SUPPORTED;
+            if (!launch) return DSPMM_STATUS_SUCCESS;
             complete_nb(c, nv, ldb, ldc, B, C, bf16);
             const Slab sl = make_slab(nv, c.vec);
             RowSeqLaunch a;
@@ -805,11 +817,16 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         }
         const bool coo = cfg.variant == DSPMM_VARIANT_COO_SCALAR || cfg.variant == DSPMM_VARIANT_COO_VECTOR;
         if (coo) {
-            dspmm_status_t s = ensure_rowidx(A);
-            if (s != DSPMM_STATUS_SUCCESS) return s;
-            s = rm ? zero_dense(C, A->nr, nv, ldc, st) : zero_dense(C, nv, A->nr, ldc, st);
+            if (!A->rowIdx && A->nnz > 0) {
+                if (!build) return DSPMM_STATUS_NOT_PREPARED;
+                dspmm_status_t s = ensure_rowidx(A);
+                if (s != DSPMM_STATUS_SUCCESS) return s;
+            }
+            if (!launch) return DSPMM_STATUS_SUCCESS;
+            dspmm_status_t s = rm ? zero_dense(C, A->nr, nv, ldc, st) : zero_dense(C, nv, A->nr, ldc, st);
             if (s != DSPMM_STATUS_SUCCESS) return s;
         }
+        if (!launch) return DSPMM_STATUS_SUCCESS;
         dspmm_status_t s = launch_compat(which, A->nr, A->nc, A->nnz, nv, ldb, ldc, A->rowPtr,
                                          A->rowIdx, A->colIdx, A->values,
                                          reinterpret_cast<const float *>(B), C, st, cfg.lanes);
@@ -838,8 +855,11 @@ dspmm_status_t check_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const voi
 dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt,
                      const dspmm_config *cfg, bool learned) {
     if (cfg && cfg->variant != DSPMM_VARIANT_AUTO) return *cfg;
-    auto it = A->tuned.find(std::make_tuple((int)layout, nv, (int)bt));
+    const auto key = std::make_tuple((int)layout, nv, (int)bt);
+    auto it = A->tuned.find(key);
     if (it != A->tuned.end()) return it->second;
+    it = A->prepared.find(key);
+    if (it != A->prepared.end()) return it->second;
     dspmm_config c;
     if (learned && layout == DENSE_ROW_MAJOR && bt == DSPMM_R_32F && learned_select(A, nv, &c))
         return c;
@@ -918,6 +938,8 @@ dspmm_status_t dspmm_matrix_create(dspmm_matrix_t *out, int nr, int nc, int nnz,
     }
     for (int i = 0; i < nnz; i++)
         if (h_colIdx[i] < 0 || h_colIdx[i] >= nc) { delete A; return DSPMM_STATUS_INVALID_VALUE; }
+    A->h_colIdx.assign(h_colIdx, h_colIdx + nnz);
+    A->h_values.assign(h_values, h_values + nnz);
     compute_stats(A);
     dspmm_status_t st = upload(&A->rowPtr, A->h_rowPtr.data(), (size_t)nr + 1, hipMemcpyHostToDevice, nullptr);
     if (st == DSPMM_STATUS_SUCCESS)
@@ -942,17 +964,27 @@ dspmm_status_t dspmm_matrix_create_device(dspmm_matrix_t *out, int nr, int nc, i
     *out = nullptr;
     if (nr < 0 || nc < 0 || nnz < 0 || !d_rowPtr) return DSPMM_STATUS_INVALID_VALUE;
     if (nnz > 0 && (!d_colIdx || !d_values)) return DSPMM_STATUS_INVALID_VALUE;
-    std::vector<int> rp((size_t)nr + 1);
+    std::vector<int> rp((size_t)nr + 1), ci((size_t)nnz);
+    std::vector<float> va((size_t)nnz);
     dspmm_status_t st = hip_status(hipStreamSynchronize(as_hip(stream)));
+    // host copies, once: plans are built from them
     if (st == DSPMM_STATUS_SUCCESS)
         st = hip_status(hipMemcpy(rp.data(), d_rowPtr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+    if (st == DSPMM_STATUS_SUCCESS && nnz)
+        st = hip_status(hipMemcpy(ci.data(), d_colIdx, sizeof(int) * ci.size(), hipMemcpyDeviceToHost));
+    if (st == DSPMM_STATUS_SUCCESS && nnz)
+        st = hip_status(hipMemcpy(va.data(), d_values, sizeof(float) * va.size(), hipMemcpyDeviceToHost));
     if (st != DSPMM_STATUS_SUCCESS) return st;
     if (!rowptr_ok(rp, nr, nnz)) return DSPMM_STATUS_INVALID_VALUE;
+    for (int i = 0; i < nnz; i++)
+        if (ci[(size_t)i] < 0 || ci[(size_t)i] >= nc) return DSPMM_STATUS_INVALID_VALUE;
     auto *A = new dspmm_matrix_s;
     A->nr = nr;
     A->nc = nc;
     A->nnz = nnz;
     A->h_rowPtr.swap(rp);
+    A->h_colIdx.swap(ci);
+    A->h_values.swap(va);
     compute_stats(A);
     const hipStream_t s = as_hip(stream);
     st = upload(&A->rowPtr, d_rowPtr, (size_t)nr + 1, hipMemcpyDeviceToDevice, s);
@@ -971,9 +1003,36 @@ dspmm_status_t dspmm_matrix_create_device(dspmm_matrix_t *out, int nr, int nc, i
 
 dspmm_status_t dspmm_matrix_destroy(dspmm_matrix_t A) {
     if (!A) return DSPMM_STATUS_SUCCESS;
-    hipDeviceSynchronize();   // no launch may still read the arrays
-    free_matrix(A);
+    free_matrix(A);       // the caller guarantees no launch still reads the handle
     return DSPMM_STATUS_SUCCESS;
+}
+
+dspmm_status_t dspmm_matrix_prepare(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                                    int64_t ldb, dspmm_dtype_t btype, const float *C, int64_t ldc,
+                                    const dspmm_config *cfg, unsigned flags) {
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (flags & ~(unsigned)DSPMM_PREPARE_HANDLE_BUFFERS) return DSPMM_STATUS_INVALID_VALUE;
+    if (A->nr == 0) return DSPMM_STATUS_SUCCESS;
+    std::lock_guard<std::mutex> lk(A->mu);
+    // sampled features (host only) computed now, not during a launch
+    (void)column_reuse(A);
+    (void)shared_row_stride(A);
+    const bool auto_cfg = !cfg || cfg->variant == DSPMM_VARIANT_AUTO;
+    const auto key = std::make_tuple((int)layout, nv, (int)btype);
+    const dspmm_config c = resolve(A, layout, nv, btype, cfg);
+    if (auto_cfg && !A->tuned.count(key)) A->prepared[key] = c;
+    const RunMode mode = (flags & DSPMM_PREPARE_HANDLE_BUFFERS) ? kRunPrepare : kRunPlans;
+    float *Cw = const_cast<float *>(C);
+    st = run_ex(A, layout, nv, B, ldb, btype, Cw, ldc, c, nullptr, 0, nullptr, nullptr, mode);
+    if (auto_cfg) {
+        // the static default AUTO falls back to when the pick cannot run
+        const dspmm_status_t sh = run_ex(A, layout, nv, B, ldb, btype, Cw, ldc,
+                                         heuristic(A, layout, nv, btype), nullptr, 0, nullptr,
+                                         nullptr, mode);
+        if (st == DSPMM_STATUS_NOT_SUPPORTED) st = sh;
+    }
+    return st;
 }
 
 dspmm_status_t dspmm_matrix_describe(dspmm_matrix_t A, dspmm_matrix_desc *d) {
@@ -1007,7 +1066,15 @@ dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout
                                             size_t *bytes) {
     if (!A || !bytes || nv < 1) return DSPMM_STATUS_INVALID_VALUE;
     std::lock_guard<std::mutex> lk(A->mu);
-    return ex_workspace_bytes(A, layout, nv, btype, resolve(A, layout, nv, btype, cfg), bytes);
+    dspmm_status_t st = ex_workspace_bytes(A, layout, nv, btype, resolve(A, layout, nv, btype, cfg),
+                                           bytes);
+    if (st != DSPMM_STATUS_SUCCESS || (cfg && cfg->variant != DSPMM_VARIANT_AUTO)) return st;
+    // AUTO may fall back to the static default when the resolved
+    // configuration cannot run these operands: cover its workspace too.
+    size_t h = 0;
+    st = ex_workspace_bytes(A, layout, nv, btype, heuristic(A, layout, nv, btype), &h);
+    *bytes = std::max(*bytes, h);
+    return st;
 }
 dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
                              int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,

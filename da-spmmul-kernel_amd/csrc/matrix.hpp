@@ -69,7 +69,10 @@ struct dspmm_matrix_s {
     int nr = 0, nc = 0, nnz = 0;
     int *rowPtr = nullptr, *rowIdx = nullptr, *colIdx = nullptr;
     float *values = nullptr;
-    std::vector<int> h_rowPtr;
+    // Host copies kept from creation: plans and sampled features are built
+    // from them, so no launch ever copies the matrix back from the device.
+    std::vector<int> h_rowPtr, h_colIdx;
+    std::vector<float> h_values;
     int max_len = 0;
     double mean = 0, cv = 0;
     size_t base_bytes = 0;
@@ -78,6 +81,7 @@ struct dspmm_matrix_s {
     double reuse16 = -1;     // sampled nonzeros per distinct column over 16-row windows (-1: not yet)
     int rb_stride = -1;      // row distance of the most shared columns, sampled (-1: not yet)
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
+    std::map<std::tuple<int, int, int>, dspmm_config> prepared;    // AUTO's pick, by dspmm_matrix_prepare
     float *carry = nullptr;
     size_t carry_bytes = 0;
     void *tbuf = nullptr;       // column-major operands transposed for the row-major kernels
@@ -94,8 +98,10 @@ struct dspmm_matrix_s {
 namespace dspmm {
 
 // All of these expect the caller to hold A->mu.
+// The plan of (S, kind, stride), built and uploaded on first use; with
+// build == false a plan not built yet is DSPMM_STATUS_NOT_PREPARED.
 dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind = kPlanNnzTiled,
-                        int stride = 1);
+                        int stride = 1, bool build = true);
 // Column reuse of neighbouring rows, sampled (computed once per matrix).
 double column_reuse(dspmm_matrix_s *A);
 // The row distance >= 2 at which up to 256 sampled rows share the most

@@ -150,6 +150,7 @@ struct dspmm_server_s {
     std::thread worker;
     bool stop = false;
     int inflight = 0;
+    std::vector<dspmm_matrix_t> evicted;                       // freed outside the mutex
     dspmm_server_stats stats{};
     int seg_cap = 0;
 
@@ -168,7 +169,10 @@ struct dspmm_server_s {
             for (auto &x : mats)
                 if (x.h && x.pins == 0 && (!victim || x.last_use < victim->last_use)) victim = &x;
             if (!victim) return false;
-            dspmm_matrix_destroy(victim->h);
+            // unpinned: no launch in flight reads it.  It is freed by the
+            // worker after it lets go of the mutex (hipFree may wait for
+            // the device), so submit / poll callers are never held up.
+            evicted.push_back(victim->h);
             victim->h = nullptr;
             cache_used -= victim->bytes;
             victim->bytes = 0;
@@ -401,6 +405,13 @@ struct dspmm_server_s {
             if (stop && queue.empty() && inflight == 0) break;
             Slot *sl = free_slot();
             if (sl && !queue.empty() && launch(*sl)) progressed = true;
+            if (!evicted.empty()) {
+                std::vector<dspmm_matrix_t> gone;
+                gone.swap(evicted);
+                lk.unlock();
+                for (dspmm_matrix_t h : gone) dspmm_matrix_destroy(h);
+                lk.lock();
+            }
             if (!progressed) {
                 if (inflight > 0) {
                     lk.unlock();
@@ -480,6 +491,7 @@ dspmm_status_t dspmm_server_destroy(dspmm_server_t s) {
     }
     for (auto &m : s->mats)
         if (m.h) dspmm_matrix_destroy(m.h);
+    for (dspmm_matrix_t h : s->evicted) dspmm_matrix_destroy(h);
     s->arena.destroy();
     delete s;
     return DSPMM_STATUS_SUCCESS;

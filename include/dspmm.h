@@ -61,7 +61,11 @@ typedef enum dspmm_status {
     DSPMM_STATUS_ALLOC_FAILED = 3,      /* device or host allocation failed */
     DSPMM_STATUS_HIP_ERROR = 4,         /* see dspmm_last_hip_error()       */
     DSPMM_STATUS_WORKSPACE_TOO_SMALL = 5,
-    DSPMM_STATUS_ADMISSION_REJECTED = 6 /* request exceeds the HBM budget   */
+    DSPMM_STATUS_ADMISSION_REJECTED = 6,/* request exceeds the HBM budget   */
+    DSPMM_STATUS_NOT_PREPARED = 7       /* a plan or buffer the call needs was
+                                           not built by dspmm_matrix_prepare, and
+                                           the call may not build it (stream
+                                           capture, dspmm_spmm_ex_noalloc)      */
 } dspmm_status_t;
 
 DSPMM_API const char *dspmm_get_error_string(dspmm_status_t status);
@@ -212,13 +216,17 @@ DSPMM_API dspmm_status_t dspmm_matrix_create(dspmm_matrix_t *A, int nr, int nc, 
                                              const int *h_colIdx, const float *h_values,
                                              unsigned flags);
 /* Same, from arrays already in device memory (copied, so the caller keeps
- * ownership of its buffers).  d_rowIdx may be NULL. */
+ * ownership of its buffers; the handle also keeps host copies, made here,
+ * from which it builds its plans).  d_rowIdx may be NULL. */
 DSPMM_API dspmm_status_t dspmm_matrix_create_device(dspmm_matrix_t *A, int nr, int nc,
                                                     int nnz, const int *d_rowPtr,
                                                     const int *d_rowIdx,
                                                     const int *d_colIdx,
                                                     const float *d_values, unsigned flags,
                                                     dspmm_stream_t stream);
+/* Frees the handle and everything it holds, without a device-wide
+ * synchronisation: the caller makes sure no launch that reads the handle
+ * is still running (as for hipFree of the arrays themselves). */
 DSPMM_API dspmm_status_t dspmm_matrix_destroy(dspmm_matrix_t A);
 
 typedef struct dspmm_matrix_desc {
@@ -296,16 +304,39 @@ typedef struct dspmm_config {
  * copies of B and C (results equal the row-major call's bit for bit); the
  * reference classes run column-major natively, FP32 only.  `workspace` may
  * be NULL: the handle's own buffers (split rows, transposed copies) are
- * then used, so calls on one handle must then not overlap in time. */
+ * then used, so calls on one handle must then not overlap in time.  Plans
+ * are built on first use unless dspmm_matrix_prepare built them. */
 DSPMM_API dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv,
                                        const void *B, int64_t ldb, dspmm_dtype_t btype,
                                        float *C, int64_t ldc, const dspmm_config *cfg,
                                        void *workspace, size_t workspace_bytes,
                                        dspmm_stream_t stream);
+/* Workspace bytes dspmm_spmm_ex needs for these arguments; for AUTO, the
+ * larger of the resolved configuration's and the static default's (the
+ * fallback when the resolved one cannot run the operands). */
 DSPMM_API dspmm_status_t dspmm_spmm_ex_workspace_size(dspmm_matrix_t A, DenseLayout layout,
                                                       int nv, dspmm_dtype_t btype,
                                                       const dspmm_config *cfg,
                                                       size_t *bytes);
+/* Builds everything a later dspmm_spmm_ex call with these arguments needs
+ * -- the tile plan of the configuration it resolves to and, for AUTO, the
+ * plan of the static default it falls back to; the row array of the COO
+ * classes -- and fixes AUTO's pick for (layout, nv, btype).  With
+ * DSPMM_PREPARE_HANDLE_BUFFERS it also sizes the handle's own split-row and
+ * transposed-copy buffers, for calls that pass no workspace.  Afterwards
+ * such a call allocates nothing, copies nothing to the host and never
+ * waits, so it may be captured into a hipGraph.  B and C are only read
+ * for their alignment.  Plans and buffers are built by blocking copies:
+ * call it outside stream capture.  An unprepared dspmm_spmm_ex builds what
+ * it needs on first use, except under stream capture, where it returns
+ * DSPMM_STATUS_NOT_PREPARED instead (no reference counterpart: the
+ * reference dispatcher, spmvspmm.h:53-199, launches from raw arrays and
+ * has no plans). */
+#define DSPMM_PREPARE_HANDLE_BUFFERS 0x1u
+DSPMM_API dspmm_status_t dspmm_matrix_prepare(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                              const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                              const float *C, int64_t ldc,
+                                              const dspmm_config *cfg, unsigned flags);
 /* Resolve what DSPMM_VARIANT_AUTO would run for these arguments. */
 DSPMM_API dspmm_status_t dspmm_spmm_ex_resolve(dspmm_matrix_t A, DenseLayout layout, int nv,
                                                dspmm_dtype_t btype, const dspmm_config *cfg,
@@ -359,8 +390,10 @@ DSPMM_API dspmm_status_t dspmm_convert_f32_to_bf16(const float *src, void *dst, 
  * Batched SpMM (no reference counterpart): many independent row-major
  * products C_i = A_i * B_i, each with its own matrix, nv and operands,
  * in ONE persistent launch (plus one launch that combines split rows).
- * Each item uses its matrix's autotuned (or heuristic) nnz-tiled config;
- * results are bit-identical to dspmm_spmm_ex with that config.  Work is
+ * Each item runs the nnz-tiled kernel with the static default vector
+ * width, lanes and tile for its nv and dense type (autotuned single-launch
+ * configurations are not used); results are bit-identical to dspmm_spmm_ex
+ * with that configuration (DSPMM_VARIANT_NNZ_TILED, other fields 0).  Work is
  * cut into wavefront units; each XCD owns a list of them, and a matrix is
  * kept on one XCD when it is small enough for its B to stay in that XCD's
  * L2.  The matrices must outlive the batch.
@@ -513,7 +546,7 @@ typedef struct dspmm_coo {
 /* Reads a coordinate Matrix-Market file (real, integer or pattern; general or
  * symmetric; 1-based indices).  Replaces readMtx<int,float>
  * (util/spmatrix.hpp:286-333) with mm_read_banner / mm_read_mtx_crd_size
- * (util/mmio.hpp:215-336), called at spmvspmm_benchmark.cu:150.  With
+ * (util/mmio.hpp:215-336), called at spmvspmm_benchmark.cu:349.  With
  * flags == 0 the result equals the reference's for the same file: symmetric
  * files are mirrored, duplicates removed, diagonal entries dropped, and the
  * compaction after a dropped entry moves indices but not values
