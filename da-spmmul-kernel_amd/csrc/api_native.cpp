@@ -904,6 +904,38 @@ float median_of(std::vector<float> v) {
 
 }  // namespace dspmm
 
+// Device scratch of dspmm_spmm_ex_noalloc (split-row carries, transposed
+// copies of column-major operands): one copy per GPU, held until the
+// process ends.  Host code never uses this name as an address: each
+// device's address comes from hipGetSymbolAddress, once, with that device
+// current (scratch_address).
+static __device__ __align__(256) unsigned char scratch[DSPMM_NOALLOC_SCRATCH_BYTES];
+
+namespace dspmm {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+dspmm_status_t scratch_address(void **out) {
+    *out = nullptr;
+    int dev = 0;
+    dspmm_status_t st = hip_status(hipGetDevice(&dev));
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (dev < 0 || dev >= kMaxDevices) return DSPMM_STATUS_NOT_SUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        st = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return DSPMM_STATUS_SUCCESS;
+}
+}  // namespace
+}  // namespace dspmm
+
 extern "C" {
 
 dspmm_status_t dspmm_fill_zero(void *ptr, size_t bytes, dspmm_stream_t stream) {
@@ -1093,6 +1125,42 @@ dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int nv, const
         st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, h, workspace, workspace_bytes,
                     as_hip(stream));
     }
+    return st;
+}
+
+dspmm_status_t dspmm_spmm_ex_noalloc(dspmm_matrix_t A, DenseLayout layout, int nv, const void *B,
+                                     int64_t ldb, dspmm_dtype_t btype, float *C, int64_t ldc,
+                                     const dspmm_config *cfg, dspmm_stream_t stream) {
+    dspmm_status_t st = check_ex(A, layout, nv, B, ldb, btype, C, ldc);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    if (A->nr == 0) return DSPMM_STATUS_SUCCESS;
+    void *ws = nullptr;
+    st = scratch_address(&ws);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    std::lock_guard<std::mutex> lk(A->mu);
+    const bool auto_cfg = !cfg || cfg->variant == DSPMM_VARIANT_AUTO;
+    dspmm_config c{};
+    if (auto_cfg) {
+        // only a configuration already resolved (tuned, or picked by
+        // dspmm_matrix_prepare): resolving afresh would compute features
+        const auto key = std::make_tuple((int)layout, nv, (int)btype);
+        auto it = A->tuned.find(key);
+        if (it != A->tuned.end()) {
+            c = it->second;
+        } else {
+            auto ip = A->prepared.find(key);
+            if (ip == A->prepared.end()) return DSPMM_STATUS_NOT_PREPARED;
+            c = ip->second;
+        }
+    } else {
+        c = *cfg;
+    }
+    const hipStream_t hs = as_hip(stream);
+    st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, c, ws, DSPMM_NOALLOC_SCRATCH_BYTES, hs,
+                nullptr, kRunNoAlloc);
+    if (st == DSPMM_STATUS_NOT_SUPPORTED && auto_cfg)
+        st = run_ex(A, layout, nv, B, ldb, btype, C, ldc, heuristic(A, layout, nv, btype), ws,
+                    DSPMM_NOALLOC_SCRATCH_BYTES, hs, nullptr, kRunNoAlloc);
     return st;
 }
 

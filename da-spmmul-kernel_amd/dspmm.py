@@ -54,6 +54,7 @@ VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
                  8: "row_blocked"}
 MATRIX_WITH_COO = 0x1
 PREPARE_HANDLE_BUFFERS = 0x1
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 STATUS_SUCCESS = 0
 STATUS_INVALID_VALUE = 1
@@ -190,6 +191,8 @@ def _declare(lib):
         "dspmm_matrix_describe": ([_P, C.POINTER(MatrixDesc)], _ST),
         "dspmm_spmm_ex": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _P, _SZ, _P],
                           _ST),
+        "dspmm_spmm_ex_noalloc": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), _P],
+                                  _ST),
         "dspmm_matrix_prepare": ([_P, _I, _I, _P, _I64, _I, _P, _I64, C.POINTER(Config), C.c_uint],
                                  _ST),
         "dspmm_spmm_ex_workspace_size": ([_P, _I, _I, _I, C.POINTER(Config), C.POINTER(_SZ)],

@@ -311,6 +311,24 @@ DSPMM_API dspmm_status_t dspmm_spmm_ex(dspmm_matrix_t A, DenseLayout layout, int
                                        float *C, int64_t ldc, const dspmm_config *cfg,
                                        void *workspace, size_t workspace_bytes,
                                        dspmm_stream_t stream);
+/* dspmm_spmm_ex that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * It calls no hipMalloc / hipFree / allocating routine: its scratch (the
+ * split-row carries and the transposed copies of column-major operands)
+ * is a static device array of DSPMM_NOALLOC_SCRATCH_BYTES that the library
+ * holds on every GPU it runs on until the process ends, and the plans and
+ * AUTO's pick must already exist: call dspmm_matrix_prepare with the same
+ * arguments first (else DSPMM_STATUS_NOT_PREPARED).  A call whose scratch
+ * would exceed DSPMM_NOALLOC_SCRATCH_BYTES returns
+ * DSPMM_STATUS_WORKSPACE_TOO_SMALL.  Every call on one GPU shares that
+ * GPU's array (each call writes what it reads): calls that overlap in time
+ * on one GPU race and must be serialised by the caller. */
+#define DSPMM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+DSPMM_API dspmm_status_t dspmm_spmm_ex_noalloc(dspmm_matrix_t A, DenseLayout layout, int nv,
+                                               const void *B, int64_t ldb, dspmm_dtype_t btype,
+                                               float *C, int64_t ldc, const dspmm_config *cfg,
+                                               dspmm_stream_t stream);
 /* Workspace bytes dspmm_spmm_ex needs for these arguments; for AUTO, the
  * larger of the resolved configuration's and the static default's (the
  * fallback when the resolved one cannot run the operands). */
