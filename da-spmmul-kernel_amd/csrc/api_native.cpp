@@ -14,6 +14,7 @@
 #include "internal.hpp"
 #include "matrix.hpp"
 #include "native.hpp"
+#include "ring.hpp"
 
 namespace dspmm {
 
@@ -154,8 +155,20 @@ using namespace dspmm;
 
 namespace dspmm {
 
+void free_ring_plan(RingPlan &p) {
+    (void)hipFree(p.d_tiles);
+    (void)hipFree(p.d_streams);
+    (void)hipFree(p.d_rows);
+    (void)hipFree(p.d_fill_off);
+    (void)hipFree(p.d_fills);
+    (void)hipFree(p.d_pslot);
+    (void)hipFree(p.d_pval);
+    p = RingPlan{};
+}
+
 void free_matrix(dspmm_matrix_s *A) {
     if (!A) return;
+    for (auto &kv : A->ring_plans) free_ring_plan(kv.second);
     hipFree(A->rowPtr);
     hipFree(A->rowIdx);
     hipFree(A->colIdx);
@@ -323,6 +336,7 @@ int shared_row_stride(dspmm_matrix_s *A) {
     // score: columns a sampled row shares with the row that far below it
     double best = 0;
     int64_t best_d = 1;
+    std::vector<std::pair<int64_t, double>> scored;
     for (const auto &f : freq) {
         const int64_t d = f.second;
         int64_t shared = 0, rows = 0;
@@ -338,13 +352,91 @@ int shared_row_stride(dspmm_matrix_s *A) {
             }
         }
         const double score = rows ? (double)shared / (double)rows : 0.0;
+        scored.emplace_back(d, score);
         if (score > best || (score == best && score > 0 && d > best_d)) {
             best = score;
             best_d = d;
         }
     }
-    if (best >= std::max(1.0, 0.25 * (double)A->nnz / (double)A->nr)) A->rb_stride = (int)best_d;
+    const double floor_score = std::max(1.0, 0.25 * (double)A->nnz / (double)A->nr);
+    if (best >= floor_score) A->rb_stride = (int)best_d;
+    // the best distance below it that divides it
+    A->rb_stride2 = 1;
+    double best2 = 0;
+    for (const auto &sc : scored) {
+        const int64_t d = sc.first;
+        if (A->rb_stride <= 1 || d >= A->rb_stride || A->rb_stride % d || sc.second < floor_score)
+            continue;
+        if (sc.second > best2 || (sc.second == best2 && d > A->rb_stride2)) {
+            best2 = sc.second;
+            A->rb_stride2 = (int)d;
+        }
+    }
     return A->rb_stride;
+}
+
+int shared_row_stride2(dspmm_matrix_s *A) {
+    if (A->rb_stride < 1) (void)shared_row_stride(A);
+    return std::max(1, A->rb_stride2);
+}
+
+dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPlan **out,
+                             bool build) {
+    const auto key = std::make_tuple(w, h, zchunk);
+    auto it = A->ring_plans.find(key);
+    if (it != A->ring_plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
+    if (!build) return DSPMM_STATUS_NOT_PREPARED;
+    RingShape sh;
+    sh.w = w;
+    sh.h = h;
+    sh.zchunk = zchunk;
+    sh.s_big = shared_row_stride(A);
+    sh.s_mid = shared_row_stride2(A);
+    if (sh.s_big > 1 && (sh.s_mid < w || sh.s_mid == 1)) {
+        // one distance only: w * h consecutive rows per tile, walked along it
+        sh.s_mid = sh.s_big;
+        sh.w = w * h;
+        sh.h = 1;
+        if (sh.w > sh.s_big) sh.s_big = 1;
+    }
+    sh.slots = 0;                   // the largest tile's distinct columns, rounded up
+    RingPlanHost hp;
+    RingPlan p;
+    p.usable = build_ring_plan(A->h_rowPtr, A->h_colIdx, A->h_values, A->nr, A->nc, sh, hp);
+    if (p.usable) {
+        p.nstreams = (int)hp.stream_off.size() - 1;
+        p.ntiles = (int)hp.tiles.size();
+        p.slots = hp.slots;
+        p.rows_max = hp.rows_max;
+        p.nnz_max = hp.nnz_max;
+        p.stream_tiles_max = hp.stream_tiles_max;
+        p.stream_fills_max = hp.stream_fills_max;
+        p.fills = hp.total_fills;
+        dspmm_status_t s = upload(&p.d_tiles, hp.tiles.data(), hp.tiles.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_streams, hp.stream_off.data(), hp.stream_off.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_rows, hp.row_info.data(), hp.row_info.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_fill_off, hp.fill_off.data(), hp.fill_off.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_fills, hp.fills.data(), hp.fills.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_pslot, hp.pslot.data(), hp.pslot.size(), hipMemcpyHostToDevice, nullptr);
+        if (s == DSPMM_STATUS_SUCCESS)
+            s = upload(&p.d_pval, hp.pval.data(), hp.pval.size(), hipMemcpyHostToDevice, nullptr);
+        if (s != DSPMM_STATUS_SUCCESS) {
+            free_ring_plan(p);
+            return s;
+        }
+        p.bytes = sizeof(int4) * hp.tiles.size() + sizeof(int) * hp.stream_off.size() +
+                  sizeof(int2) * hp.row_info.size() + sizeof(int) * hp.fill_off.size() +
+                  sizeof(int2) * hp.fills.size() + sizeof(uint16_t) * hp.pslot.size() +
+                  sizeof(float) * hp.pval.size();
+    }
+    auto res = A->ring_plans.emplace(key, p);
+    *out = &res.first->second;
+    return DSPMM_STATUS_SUCCESS;
 }
 
 dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int stride,
@@ -472,7 +564,8 @@ bool rm_native(const dspmm_config &c, bool bf16) {
     switch (c.variant) {
     case DSPMM_VARIANT_NNZ_TILED:
     case DSPMM_VARIANT_ROW_BLOCKED:
-    case DSPMM_VARIANT_NNZ_CACHED: return true;
+    case DSPMM_VARIANT_NNZ_CACHED:
+    case DSPMM_VARIANT_RING: return true;
     case DSPMM_VARIANT_CSR_SCALAR: return c.unroll > 0 || bf16;
     default: return false;
     }
@@ -730,6 +823,52 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         a.unroll = cfg.unroll > 0 ? cfg.unroll : 8;
         a.bf16 = bf16;
         s = launch_row_blocked(a, st);
+        if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
+        return s;
+    }
+    case DSPMM_VARIANT_RING: {
+        // (an empty matrix has nothing to stage: the other kernels take it)
+        if (layout != DENSE_ROW_MAJOR || A->nnz == 0) return DSPMM_STATUS_NOT_SUPPORTED;
+        const int w = cfg.tile_nnz > 0 ? cfg.tile_nnz : 8;
+        const int h = cfg.unroll > 0 ? cfg.unroll : 4;
+        const int z = cfg.row_stride > 0 ? cfg.row_stride : 25;
+        if (w > 256 || h > 256 || w * h > 256) return DSPMM_STATUS_NOT_SUPPORTED;
+        dspmm_config c = cfg;
+        c.tile_nnz = 0;
+        c.unroll = 0;
+        complete_nb(c, nv, ldb, ldc, B, C, bf16);
+        const Slab sl = make_slab(nv, c.vec);
+        if (sl.count != 1) return DSPMM_STATUS_NOT_SUPPORTED;          // one slab only
+        if (A->rb_stride < 1 && !build) return DSPMM_STATUS_NOT_PREPARED;
+        RingPlan *plan = nullptr;
+        dspmm_status_t s = get_ring_plan(A, w, h, z, &plan, build);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
+        RingLaunch a;
+        a.tiles = plan->d_tiles;
+        a.stream_off = plan->d_streams;
+        a.row_info = plan->d_rows;
+        a.fill_off = plan->d_fill_off;
+        a.fills = plan->d_fills;
+        a.pslot = plan->d_pslot;
+        a.pval = plan->d_pval;
+        a.B = B;
+        a.C = C;
+        a.ldb = ldb;
+        a.ldc = ldc;
+        a.nv = nv;
+        a.nstreams = plan->nstreams;
+        a.slots = plan->slots;
+        a.rows_max = plan->rows_max;
+        a.nnz_max = plan->nnz_max;
+        a.stream_tiles_max = plan->stream_tiles_max;
+        a.stream_fills_max = plan->stream_fills_max;
+        a.vw = c.vec;
+        a.lanes = sl.lanes;
+        a.bf16 = bf16;
+        if (!ring_supported(a)) return DSPMM_STATUS_NOT_SUPPORTED;
+        if (!launch) return DSPMM_STATUS_SUCCESS;
+        s = launch_ring(a, st);
         if (s == DSPMM_STATUS_SUCCESS && mid) s = hip_status(hipEventRecord(mid, st));
         return s;
     }
@@ -1284,6 +1423,19 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                     c.row_stride = rs;
                     cand.push_back(c);
                 }
+        if (stride > 1 || reuse >= kRowBlockMinReuse) {
+            // B-row ring in LDS: tile shapes (w x h) and tiles per stream
+            for (const auto &wh : {std::make_pair(8, 4), std::make_pair(8, 8),
+                                   std::make_pair(16, 4), std::make_pair(4, 8)})
+                for (int z : {10, 25}) {
+                    dspmm_config c{};
+                    c.variant = DSPMM_VARIANT_RING;
+                    c.tile_nnz = wh.first;
+                    c.unroll = wh.second;
+                    c.row_stride = z;
+                    cand.push_back(c);
+                }
+        }
         if (reuse >= kCachedMinReuse) {
             dspmm_config base{};
             base.variant = DSPMM_VARIANT_NNZ_TILED;
@@ -1443,7 +1595,7 @@ dspmm_status_t dspmm_matrix_set_config(dspmm_matrix_t A, DenseLayout layout, int
                                        dspmm_dtype_t btype, const dspmm_config *cfg) {
     if (!A || !cfg || nv < 1 || (layout != DENSE_ROW_MAJOR && layout != DENSE_COL_MAJOR) ||
         (btype != DSPMM_R_32F && btype != DSPMM_R_16BF) || cfg->variant < DSPMM_VARIANT_AUTO ||
-        cfg->variant > DSPMM_VARIANT_ROW_BLOCKED)
+        cfg->variant > DSPMM_VARIANT_RING)
         return DSPMM_STATUS_INVALID_VALUE;
     std::lock_guard<std::mutex> lk(A->mu);
     const auto key = std::make_tuple((int)layout, nv, (int)btype);

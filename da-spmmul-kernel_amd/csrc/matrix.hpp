@@ -31,6 +31,22 @@ struct NbPlan {
     bool usable = true;               // false: rows unsorted or with repeated columns
 };
 
+// A B-row ring plan (k_ring) uploaded to the device.
+struct RingPlan {
+    int4 *d_tiles = nullptr;
+    int *d_streams = nullptr;
+    int2 *d_rows = nullptr;
+    int *d_fill_off = nullptr;
+    int2 *d_fills = nullptr;
+    uint16_t *d_pslot = nullptr;
+    float *d_pval = nullptr;
+    int nstreams = 0, ntiles = 0, slots = 0, rows_max = 0, nnz_max = 0;
+    int stream_tiles_max = 0, stream_fills_max = 0;
+    int64_t fills = 0;
+    size_t bytes = 0;
+    bool usable = false;     // false: a tile needs more B rows than the ring holds
+};
+
 // LDS budget of one nnz-tiled workgroup (keeps two workgroups per CU).
 constexpr size_t kMaxTileLds = 64 * 1024;
 
@@ -80,6 +96,9 @@ struct dspmm_matrix_s {
     std::map<int64_t, dspmm::NbPlan> plans;                        // by 4 * (tile size + 16 * (stride - 1)) + kind
     double reuse16 = -1;     // sampled nonzeros per distinct column over 16-row windows (-1: not yet)
     int rb_stride = -1;      // row distance of the most shared columns, sampled (-1: not yet)
+    int rb_stride2 = -1;     // the best distance below it that divides it (1: none)
+    // ring plans by (w, h, tiles per stream)
+    std::map<std::tuple<int, int, int>, dspmm::RingPlan> ring_plans;
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     std::map<std::tuple<int, int, int>, dspmm_config> prepared;    // AUTO's pick, by dspmm_matrix_prepare
     float *carry = nullptr;
@@ -91,6 +110,7 @@ struct dspmm_matrix_s {
         size_t b = base_bytes + carry_bytes + tbuf_bytes;
         if (rowIdx) b += sizeof(int) * (size_t)nnz;
         for (auto &kv : plans) b += kv.second.bytes;
+        for (auto &kv : ring_plans) b += kv.second.bytes;
         return b;
     }
 };
@@ -110,6 +130,14 @@ double column_reuse(dspmm_matrix_s *A);
 // when they share fewer than max(1, mean row length / 4) on average.
 // Computed once per matrix.
 int shared_row_stride(dspmm_matrix_s *A);
+// The distance below shared_row_stride(A) that divides it and at which the
+// sampled rows share the most columns (a stencil's line length below its
+// plane length), or 1 when none qualifies.  Computed with it.
+int shared_row_stride2(dspmm_matrix_s *A);
+// The ring plan of tile shape (w, h) and `zchunk` tiles per stream, built
+// on first use (build == false: DSPMM_STATUS_NOT_PREPARED if missing).
+dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPlan **out,
+                             bool build = true);
 int cached_smax(int umax);
 int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,

@@ -49,9 +49,10 @@ VARIANT_CSR_SCALAR, VARIANT_CSR_VECTOR, VARIANT_COO_SCALAR, VARIANT_COO_VECTOR =
 VARIANT_SPMV_TILED = 6
 VARIANT_NNZ_CACHED = 7
 VARIANT_ROW_BLOCKED = 8
+VARIANT_RING = 9
 VARIANT_NAMES = {0: "auto", 1: "nnz_tiled", 2: "csr_scalar", 3: "csr_vector",
                  4: "coo_scalar", 5: "coo_vector", 6: "spmv_tiled", 7: "nnz_cached",
-                 8: "row_blocked"}
+                 8: "row_blocked", 9: "ring"}
 MATRIX_WITH_COO = 0x1
 PREPARE_HANDLE_BUFFERS = 0x1
 NOALLOC_SCRATCH_BYTES = 64 << 20

@@ -272,13 +272,21 @@ typedef enum dspmm_variant {
                                     tile_nnz = distinct columns per tile,
                                     unroll = workgroup size (64/128/256,
                                     0 = from the plan's rows per tile) */
-    DSPMM_VARIANT_ROW_BLOCKED = 8  /* row-major, sorted rows without repeated
+    DSPMM_VARIANT_ROW_BLOCKED = 8, /* row-major, sorted rows without repeated
                                     columns: a lane group owns tile_nnz = 2, 4
                                     or 8 rows, row_stride apart (consecutive
                                     by default), and loads each B row their
                                     columns share once; rows are summed like
                                     the host loop (row-exact); unroll = B rows
                                     in flight (4 or 8) */
+    DSPMM_VARIANT_RING = 9       /* row-major, nv a multiple of 4 (FP32) or 8
+                                    (BF16) fitting one slab: B rows staged in a
+                                    ring in LDS whose fills and evictions a host
+                                    plan decides; a workgroup walks tiles of
+                                    tile_nnz consecutive rows x unroll runs
+                                    (the matrix's sampled row distances apart),
+                                    row_stride tiles per walk (0 = 25); rows
+                                    are summed like the host loop (row-exact) */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {

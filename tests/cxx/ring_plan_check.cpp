@@ -1,0 +1,232 @@
+// Host-side check of the B-row ring plan (csrc/ring_plan.cpp), no GPU:
+// replays every stream's fills into a simulated ring and checks that each
+// nonzero's slot holds its column when its tile runs, that the plan's
+// values and row ends follow the CSR arrays, that every row appears once,
+// and that the limits the kernel relies on hold (slots < capacity, tiles'
+// nonzero ranges 8-aligned).  Then it evaluates C through the plan in the
+// kernel's order and compares it bit for bit with the row-by-row host loop.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ring.hpp"
+
+using dspmm::RingPlanHost;
+using dspmm::RingShape;
+
+struct Csr {
+    int nr = 0, nc = 0;
+    std::vector<int> rp, ci;
+    std::vector<float> va;
+};
+
+static float hval(uint64_t i) {
+    uint64_t z = i * 0x9E3779B97F4A7C15ull + 12345;
+    z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+    return (float)((z >> 40) & 0xffff) / 65536.0f - 0.5f;
+}
+
+// 27-point stencil on an n^3 grid, x fastest.
+static Csr stencil(int n) {
+    Csr a;
+    a.nr = a.nc = n * n * n;
+    a.rp.push_back(0);
+    for (int z = 0; z < n; z++)
+        for (int y = 0; y < n; y++)
+            for (int x = 0; x < n; x++) {
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const int xx = x + dx, yy = y + dy, zz = z + dz;
+                            if (xx < 0 || yy < 0 || zz < 0 || xx >= n || yy >= n || zz >= n) continue;
+                            a.ci.push_back(xx + n * (yy + n * zz));
+                            a.va.push_back(hval(a.ci.size()));
+                        }
+                a.rp.push_back((int)a.ci.size());
+            }
+    return a;
+}
+
+// Random rows of 0..maxlen sorted distinct columns (some empty).
+static Csr random_rows(int nr, int nc, int maxlen, uint64_t seed) {
+    Csr a;
+    a.nr = nr;
+    a.nc = nc;
+    a.rp.push_back(0);
+    uint64_t s = seed;
+    auto next = [&]() { s = s * 6364136223846793005ull + 1442695040888963407ull; return s >> 33; };
+    std::vector<char> used((size_t)nc, 0);
+    for (int r = 0; r < nr; r++) {
+        const int len = (int)(next() % (uint64_t)(maxlen + 1));
+        std::vector<int> cols;
+        for (int k = 0; k < len; k++) {
+            const int c = (int)(next() % (uint64_t)nc);
+            if (!used[(size_t)c]) { used[(size_t)c] = 1; cols.push_back(c); }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int c : cols) {
+            used[(size_t)c] = 0;
+            a.ci.push_back(c);
+            a.va.push_back(hval(a.ci.size() + seed));
+        }
+        a.rp.push_back((int)a.ci.size());
+    }
+    return a;
+}
+
+static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
+    RingPlanHost p;
+    if (!dspmm::build_ring_plan(a.rp, a.ci, a.va, a.nr, a.nc, sh, p)) {
+        std::printf("%s: plan refused\n", name);
+        return 1;
+    }
+    const int U = p.slots;
+    const int ntiles = (int)p.tiles.size();
+    std::vector<char> seen((size_t)a.nr, 0);
+    // B and the host loop's C
+    std::vector<float> B((size_t)a.nc * nv), Chost((size_t)a.nr * nv, 0.f), Cplan((size_t)a.nr * nv, -1.f);
+    for (size_t i = 0; i < B.size(); i++) B[i] = hval(i * 7 + 3);
+    for (int r = 0; r < a.nr; r++)
+        for (int k = 0; k < nv; k++) {
+            float acc = 0.f;
+            for (int q = a.rp[(size_t)r]; q < a.rp[(size_t)r + 1]; q++)
+                acc = acc + a.va[(size_t)q] * B[(size_t)a.ci[(size_t)q] * nv + k];
+            Chost[(size_t)r * nv + k] = acc;
+        }
+    int64_t fills = 0;
+    for (size_t st = 0; st + 1 < p.stream_off.size(); st++) {
+        std::vector<int> ring((size_t)U, -1);
+        for (int t = p.stream_off[st]; t < p.stream_off[st + 1]; t++) {
+            for (int f = p.fill_off[(size_t)t]; f < p.fill_off[(size_t)t + 1]; f++) {
+                const int2 fl = p.fills[(size_t)f];
+                if (fl.y < 0 || fl.y >= U || fl.x < 0 || fl.x >= a.nc) {
+                    std::printf("%s: fill out of range\n", name);
+                    return 2;
+                }
+                ring[(size_t)fl.y] = fl.x;
+                fills++;
+            }
+            const int4 td = p.tiles[(size_t)t];
+            if (td.x % 8 || td.y % 8 || td.y - td.x > p.nnz_max || td.w - td.z > p.rows_max) {
+                std::printf("%s: tile %d range\n", name, t);
+                return 3;
+            }
+            int prev = 0;
+            for (int q = td.z; q < td.w; q++) {
+                const int2 ri = p.row_info[(size_t)q];
+                const int r = ri.x;
+                if (r < 0 || r >= a.nr || seen[(size_t)r]) { std::printf("%s: row %d\n", name, r); return 4; }
+                seen[(size_t)r] = 1;
+                const int len = a.rp[(size_t)r + 1] - a.rp[(size_t)r];
+                const int padded = ri.y - prev;
+                // every row of the tile padded to the same multiple of 4
+                const int first = p.row_info[(size_t)td.z].y;
+                if (padded % 4 || padded < len || padded != first || td.x + ri.y > td.y) {
+                    std::printf("%s: row %d end\n", name, r);
+                    return 5;
+                }
+                for (int k = 0; k < nv; k++) {
+                    // the kernel's order: the row's nonzeros, then its
+                    // padding (zero slot, value 0) -- 0 * 0 added to the sum
+                    float acc = 0.f;
+                    for (int e = 0; e < padded; e++) {
+                        const int pe = td.x + prev + e;
+                        const int slot = p.pslot[(size_t)pe];
+                        if (e >= len) {
+                            if (slot != p.zero_slot || p.pval[(size_t)pe] != 0.f) {
+                                std::printf("%s: row %d padding entry %d\n", name, r, e);
+                                return 10;
+                            }
+                            acc = acc + p.pval[(size_t)pe] * 0.f;
+                            continue;
+                        }
+                        const int col = a.ci[(size_t)a.rp[(size_t)r] + e];
+                        if (slot >= U || ring[(size_t)slot] != col || p.pval[(size_t)pe] != a.va[(size_t)a.rp[(size_t)r] + e]) {
+                            std::printf("%s: tile %d row %d nonzero %d: slot %d holds %d, wants %d\n", name, t,
+                                        r, e, slot, slot < U ? ring[(size_t)slot] : -9, col);
+                            return 6;
+                        }
+                        acc = acc + p.pval[(size_t)pe] * B[(size_t)ring[(size_t)slot] * nv + k];
+                    }
+                    Cplan[(size_t)r * nv + k] = acc;
+                }
+                prev = ri.y;
+            }
+        }
+    }
+    for (int r = 0; r < a.nr; r++)
+        if (!seen[(size_t)r]) { std::printf("%s: row %d missing\n", name, r); return 7; }
+    if (std::memcmp(Chost.data(), Cplan.data(), sizeof(float) * Chost.size()) != 0) {
+        std::printf("%s: C through the plan differs from the host loop\n", name);
+        return 8;
+    }
+    if (fills != p.total_fills || p.zero_slot != U) return 9;
+    std::printf("%s: ok, %d streams, %d tiles, %d slots, %.3f fills per row, nnz_max %d, rows_max %d\n",
+                name, (int)p.stream_off.size() - 1, ntiles, U, (double)fills / std::max(1, a.nr),
+                p.nnz_max, p.rows_max);
+    return 0;
+}
+
+int main() {
+    int rc = 0;
+    {
+        const int n = 21;                      // w does not divide the line length
+        Csr a = stencil(n);
+        for (int w : {8, 4, 16})
+            for (int h : {4, 8})
+                for (int z : {3, 10}) {
+                    RingShape sh;
+                    sh.w = w;
+                    sh.h = h;
+                    sh.s_mid = n;
+                    sh.s_big = n * n;
+                    sh.zchunk = z;
+                    char name[64];
+                    std::snprintf(name, sizeof name, "stencil%d w%d h%d z%d", n, w, h, z);
+                    rc |= check(name, a, sh, 8);
+                }
+        // a fixed, larger ring: Belady keeps more than the halo
+        RingShape sh;
+        sh.s_mid = n;
+        sh.s_big = n * n;
+        sh.slots = 400;
+        rc |= check("stencil21 w8 h4 z25 U400", a, sh, 4);
+        // one distance only: tiles of consecutive rows walked along it
+        RingShape s1;
+        s1.w = 16;
+        s1.h = 1;
+        s1.s_mid = n * n;
+        s1.s_big = n * n;
+        s1.zchunk = 7;
+        rc |= check("stencil21 one distance", a, s1, 4);
+    }
+    {
+        Csr a = random_rows(3000, 5000, 40, 11);     // no structure: consecutive tiles
+        RingShape sh;
+        sh.w = 8;
+        sh.h = 4;
+        sh.zchunk = 5;
+        rc |= check("random consecutive", a, sh, 4);
+        sh.slots = 2048;                             // roomy ring
+        rc |= check("random consecutive U2048", a, sh, 4);
+    }
+    {
+        Csr a = stencil(12);
+        RingShape sh;                                // more slots than a tile needs
+        sh.s_mid = 12;
+        sh.s_big = 144;
+        sh.w = 4;
+        sh.h = 4;
+        sh.zchunk = 12;
+        rc |= check("stencil12 w4 h4", a, sh, 16);
+        sh.slots = 40;                               // fewer than a tile needs: refused
+        RingPlanHost p;
+        if (dspmm::build_ring_plan(a.rp, a.ci, a.va, a.nr, a.nc, sh, p)) {
+            std::printf("a ring smaller than a tile's columns must be refused\n");
+            rc |= 1;
+        }
+    }
+    return rc;
+}

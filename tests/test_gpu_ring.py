@@ -1,0 +1,146 @@
+"""GPU parity of the B-row ring kernel (DSPMM_VARIANT_RING): every row is
+summed in nonzero order with a separately rounded multiply and add, so every
+output equals the reference host loop bit for bit -- FP32 and BF16-stored B,
+grid stencils with two row distances (line and plane), with one distance,
+matrices without structure (consecutive tiles), several tile shapes and walk
+lengths, column-major operands through transposed copies.  Operands the
+kernel does not take (nv not a multiple of the vector width, a tile with
+more nonzeros than one register load of metadata) are refused and AUTO falls
+back."""
+import numpy as np
+import pytest
+
+import dspmm as D
+import oracle as O
+import workloads as W
+from gpu_common import Case, assert_bitwise, bits, case, note
+
+pytestmark = pytest.mark.gpu
+RM, CM = D.DENSE_ROW_MAJOR, D.DENSE_COL_MAJOR
+_MATS = {}
+
+
+def stencil2d(n, seed=7):
+    """9-point stencil on an n x n grid (one shared row distance, n)."""
+    rows, cols = [], []
+    for y in range(n):
+        for x in range(n):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    xx, yy = x + dx, y + dy
+                    if 0 <= xx < n and 0 <= yy < n:
+                        rows.append(y * n + x)
+                        cols.append(yy * n + xx)
+    rows = np.asarray(rows)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n * n))]).astype(np.int32)
+    v = np.random.default_rng(seed).uniform(0.5, 1.5, len(cols)).astype(np.float32)
+    return Case(f"stencil2d_{n}", n * n, n * n, rp, np.asarray(cols, np.int32), v, 64, seed=seed)
+
+
+def matrix(name):
+    if name not in _MATS:
+        c = stencil2d(120) if name == "stencil2d" else case(name)
+        _MATS[name] = (c, D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values))
+    return _MATS[name]
+
+
+def run(c, m, nv, cfg, layout=RM, btype=D.DSPMM_R_32F, B=None):
+    c.fill_output(np.nan)
+    m.spmm(B if B is not None else c.d_B, c.d_C, nv, layout=layout, config=cfg, btype=btype)
+    D.synchronize()
+    return c.output(nv)
+
+
+SHAPES = [(8, 4, 25), (8, 8, 10), (4, 4, 3), (16, 4, 25)]
+
+
+@pytest.mark.parametrize("name", ["stencil_20", "stencil2d", "c1"])
+@pytest.mark.parametrize("w,h,z", SHAPES)
+def test_ring_bit_exact(name, w, h, z):
+    c, m = matrix(name)
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=w, unroll=h, row_stride=z)
+    ran = 0
+    for nv in (4, 8, 16, 32, 64):
+        if nv > c.maxnv:
+            continue
+        what = f"{name} ring w={w} h={h} z={z} nv={nv}"
+        note(what)
+        try:
+            out = run(c, m, nv, cfg)
+        except D.DspmmError as e:
+            # a tile of an unstructured matrix needs many B rows: wide rows
+            # may not fit LDS; grid stencils always fit
+            assert e.status == D.STATUS_NOT_SUPPORTED and name == "c1", what
+            note(f"{what}: ring does not fit LDS")
+            continue
+        h32, _, _ = c.ref(RM, nv)
+        assert_bitwise(out, h32, what)
+        ran += 1
+    # grid stencils run at every width; an unstructured matrix's tiles and
+    # walks may need more LDS than a CU has (refused: AUTO falls back)
+    assert ran >= 3 or name == "c1"
+    # BF16-stored B: the host loop on the rounded B, bit for bit
+    d16 = D.to_bf16_device(c.d_B)
+    for nv in (8, 64):
+        if nv > c.maxnv:
+            continue
+        try:
+            out = run(c, m, nv, cfg, btype=D.DSPMM_R_16BF, B=d16)
+        except D.DspmmError as e:
+            assert e.status == D.STATUS_NOT_SUPPORTED and name == "c1"
+            continue
+        h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                             O.round_bf16(c.dense(nv))).reshape(-1)
+        assert_bitwise(out, h32r, f"{name} ring w={w} h={h} bf16 nv={nv}")
+
+
+def test_ring_integer_exact_and_column_major():
+    c, m = matrix("int_valued")          # its 4000-long row does not fit a tile's metadata
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=2, row_stride=8)
+    with pytest.raises(D.DspmmError) as e:
+        run(c, m, 32, cfg)
+    assert e.value.status == D.STATUS_NOT_SUPPORTED
+    # AUTO never lands on an unsupported configuration
+    out = run(c, m, 32, None)
+    _, c64, _ = c.ref(RM, 32)
+    assert np.array_equal(out, c64.astype(np.float32))
+    # column-major operands through transposed copies: the row-major bits
+    s, ms = matrix("stencil_20")
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=8, unroll=4, row_stride=10)
+    rm = run(s, ms, 32, cfg).copy()
+    Bcm = np.ascontiguousarray(s.dense(32).reshape(s.nc, 32).T)
+    d_Bcm = D.DeviceArray.from_numpy(Bcm)
+    s.fill_output(np.nan)
+    ms.spmm(d_Bcm, s.d_C, 32, layout=CM, config=cfg)
+    D.synchronize()
+    cm = s.output(32).reshape(32, s.nr).T.reshape(-1)
+    assert np.array_equal(bits(cm), bits(rm)), "column-major ring differs from row-major"
+
+
+def test_ring_runs_unstructured_small_tiles():
+    c, m = matrix("c1")
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=4, row_stride=3)
+    for nv in (4, 16, 32):
+        out = run(c, m, nv, cfg)
+        h32, _, _ = c.ref(RM, nv)
+        assert_bitwise(out, h32, f"c1 ring 4x4 z3 nv={nv}")
+
+
+def test_ring_refuses_what_it_cannot_take():
+    c, m = matrix("stencil_20")
+    cfg = D.make_config(D.VARIANT_RING)
+    for nv in (1, 2, 6):                 # not whole 16-byte pieces per B row
+        with pytest.raises(D.DspmmError) as e:
+            run(c, m, nv, cfg)
+        assert e.value.status == D.STATUS_NOT_SUPPORTED, nv
+
+
+def test_autotuner_times_ring_candidates():
+    c, m = matrix("stencil_20")
+    tab = m.autotune_table(c.d_B, c.d_C, 64, reps=2)
+    ring = [(cfg.as_dict(), ms) for cfg, ms in tab if cfg.variant == D.VARIANT_RING]
+    assert ring and all(ms is not None for _, ms in ring), ring
+    best = min(ring, key=lambda x: x[1])
+    others = min(ms for cfg, ms in tab if ms is not None and cfg.variant != D.VARIANT_RING)
+    note(f"stencil_20 nv=64: best ring {best[1] * 1e3:.1f} us {best[0]}, best other "
+         f"{others * 1e3:.1f} us")
