@@ -411,7 +411,10 @@ dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPl
         p.nnz_max = hp.nnz_max;
         p.stream_tiles_max = hp.stream_tiles_max;
         p.stream_fills_max = hp.stream_fills_max;
+        p.fill_max = hp.fill_max;
         p.fills = hp.total_fills;
+        p.entries = (int64_t)hp.pslot.size();
+        p.fill_entries = (int64_t)hp.fills.size();
         dspmm_status_t s = upload(&p.d_tiles, hp.tiles.data(), hp.tiles.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
             s = upload(&p.d_streams, hp.stream_off.data(), hp.stream_off.size(), hipMemcpyHostToDevice, nullptr);
@@ -863,6 +866,11 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         a.nnz_max = plan->nnz_max;
         a.stream_tiles_max = plan->stream_tiles_max;
         a.stream_fills_max = plan->stream_fills_max;
+        a.fill_max = plan->fill_max;
+        a.nr = A->nr;
+        a.nc = A->nc;
+        a.plan_entries = plan->entries;
+        a.plan_fills = plan->fill_entries;
         a.vw = c.vec;
         a.lanes = sl.lanes;
         a.bf16 = bf16;

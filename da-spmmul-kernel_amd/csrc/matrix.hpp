@@ -41,8 +41,9 @@ struct RingPlan {
     uint16_t *d_pslot = nullptr;
     float *d_pval = nullptr;
     int nstreams = 0, ntiles = 0, slots = 0, rows_max = 0, nnz_max = 0;
-    int stream_tiles_max = 0, stream_fills_max = 0;
+    int stream_tiles_max = 0, stream_fills_max = 0, fill_max = 0;
     int64_t fills = 0;
+    int64_t entries = 0, fill_entries = 0;   // lengths of d_pslot / d_pval and of d_fills
     size_t bytes = 0;
     bool usable = false;     // false: a tile needs more B rows than the ring holds
 };

@@ -67,14 +67,23 @@ struct RingLaunch {
     int slots = 0;
     int rows_max = 0, nnz_max = 0;
     int stream_tiles_max = 0, stream_fills_max = 0;
+    int fill_max = 0;                 // fills of the largest tile after a stream's first
+    int nr = 0, nc = 0;               // rows of C and of B
+    int64_t plan_entries = 0;         // entries of pslot / pval
+    int64_t plan_fills = 0;           // entries of fills
     int vw = 4, lanes = 16;
     bool bf16 = false;
 };
-// LDS bytes of one k_ring workgroup: the ring, one tile's metadata and
-// rows, and its stream's tile descriptors and fill list.
+// LDS bytes of one k_ring workgroup: the ring, two tiles' metadata and
+// rows, and its stream's tile descriptors and fill offsets.
 size_t ring_lds_bytes(const RingLaunch &a);
-// Elements per ring row: nv plus one 16-byte pad.
-__host__ __device__ inline int ring_row_elems(int nv, bool bf16) { return nv + (bf16 ? 8 : 4); }
+// Elements per ring row: nv, plus a 16-byte pad for rows under 128 bytes
+// (there lane groups of under eight lanes share a 16-byte-per-lane LDS
+// read, and rows a power of two apart would meet in the same banks).
+__host__ __device__ inline int ring_row_elems(int nv, bool bf16) {
+    const int eb = bf16 ? 2 : 4;
+    return nv + (nv * eb >= 128 ? 0 : 16 / eb);
+}
 // Whether these operands and this plan fit the kernel (one slab, 16-byte
 // pieces, a tile's metadata in one register load, the LDS budget).
 bool ring_supported(const RingLaunch &a);

@@ -15,7 +15,11 @@
 // Per stream the ring (U slots, one B row each) is simulated with Belady's
 // rule: a B row a tile needs and the ring lacks is loaded into a free slot
 // or into the slot of the resident row whose next use in the stream is
-// farthest away (never used again first).  The plan lists, per tile, the
+// farthest away (never used again first), among the rows that neither
+// this tile nor the one before it reads: the kernel writes a tile's fills
+// while its waves may still be computing the previous tile.  The ring
+// holds, by default, the most distinct B rows two consecutive tiles of a
+// stream need (rounded up to 8).  The plan lists, per tile, the
 // (column, slot) fills to perform before the tile runs, and gives every
 // nonzero the slot its B row occupies when its tile runs.  Each row keeps
 // its nonzeros in storage order, so the kernel sums every row exactly like
@@ -127,8 +131,23 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
             need_off[(size_t)t + 1] = (int)need.size();
         }
     }
-    const int slots = sh.slots > 0 ? sh.slots : std::max(8, (need_max + 7) / 8 * 8);
-    if (need_max > slots || slots > 65534) return false;           // a tile must fit the ring
+    // the most distinct columns of two consecutive tiles of a stream
+    int pair_max = need_max;
+    for (int s = 0; s < nstreams; s++)
+        for (int t = o.stream_off[(size_t)s] + 1; t < o.stream_off[(size_t)s + 1]; t++) {
+            const auto a0 = need.begin() + need_off[(size_t)t - 1], a1 = need.begin() + need_off[(size_t)t];
+            const auto b1 = need.begin() + need_off[(size_t)t + 1];
+            // |A u B| = |A| + |B| - |A n B| over the sorted sets
+            int common = 0;
+            for (auto x = a0, y = a1; x != a1 && y != b1;) {
+                if (*x < *y) ++x;
+                else if (*y < *x) ++y;
+                else { ++common; ++x; ++y; }
+            }
+            pair_max = std::max(pair_max, (int)(a1 - a0) + (int)(b1 - a1) - common);
+        }
+    const int slots = sh.slots > 0 ? sh.slots : std::max(8, (pair_max + 7) / 8 * 8);
+    if (pair_max > slots || slots > 65534) return false;   // two consecutive tiles must fit
     p.slots = slots;
 
     // Belady simulation, stream by stream
@@ -164,14 +183,14 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
                 if (slot_of_col[(size_t)c] < 0) missing.push_back(i);
             }
             // victims: free slots first, then the farthest next use among
-            // residents this tile does not need
+            // residents neither this tile nor the previous one needs
             int nfree = slots - free_next;
             if ((int)missing.size() > nfree) {
                 cand.clear();
                 for (int u = 0; u < free_next; u++) {
                     const int c = col_of_slot[(size_t)u];
                     if (c < 0) { cand.emplace_back(INT_MAX, u); continue; }
-                    if (needed_at[(size_t)c] == t) continue;      // this tile reads it
+                    if (needed_at[(size_t)c] >= std::max(t - 1, t0)) continue;  // read by t or t - 1
                     const int h = occ_head[(size_t)c];
                     cand.emplace_back(h < 0 ? INT_MAX : occ_tile[(size_t)h], u);
                 }

@@ -1,10 +1,13 @@
 // Host-side check of the B-row ring plan (csrc/ring_plan.cpp), no GPU:
 // replays every stream's fills into a simulated ring and checks that each
-// nonzero's slot holds its column when its tile runs, that the plan's
-// values and row ends follow the CSR arrays, that every row appears once,
-// and that the limits the kernel relies on hold (slots < capacity, tiles'
-// nonzero ranges 8-aligned).  Then it evaluates C through the plan in the
+// nonzero's slot holds its column when its tile runs, that no fill of a
+// tile overwrites a slot the previous tile of its stream reads (the kernel
+// writes a tile's fills while the previous tile may still be computing),
+// that the plan's values and row ends follow the CSR arrays, that every row
+// appears once, and that the limits the kernel relies on hold (slots <
+// capacity, tiles' nonzero ranges 8-aligned).  Then it evaluates C through the plan in the
 // kernel's order and compares it bit for bit with the row-by-row host loop.
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -98,12 +101,18 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
     int64_t fills = 0;
     for (size_t st = 0; st + 1 < p.stream_off.size(); st++) {
         std::vector<int> ring((size_t)U, -1);
+        std::vector<int> last_read((size_t)U, INT_MIN); // last tile of the stream reading a slot
         for (int t = p.stream_off[st]; t < p.stream_off[st + 1]; t++) {
             for (int f = p.fill_off[(size_t)t]; f < p.fill_off[(size_t)t + 1]; f++) {
                 const int2 fl = p.fills[(size_t)f];
                 if (fl.y < 0 || fl.y >= U || fl.x < 0 || fl.x >= a.nc) {
                     std::printf("%s: fill out of range\n", name);
                     return 2;
+                }
+                if (last_read[(size_t)fl.y] >= t - 1) {
+                    std::printf("%s: tile %d fills slot %d, which tile %d reads\n", name, t, fl.y,
+                                last_read[(size_t)fl.y]);
+                    return 11;
                 }
                 ring[(size_t)fl.y] = fl.x;
                 fills++;
@@ -149,6 +158,7 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
                             return 6;
                         }
                         acc = acc + p.pval[(size_t)pe] * B[(size_t)ring[(size_t)slot] * nv + k];
+                        last_read[(size_t)slot] = t;
                     }
                     Cplan[(size_t)r * nv + k] = acc;
                 }
