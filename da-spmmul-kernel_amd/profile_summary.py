@@ -114,6 +114,18 @@ def derived(v, cus, xcds):
                                              out["L1-miss latency (cycles)"])
     if v.get("SQ_INSTS_VMEM_RD") and v.get("SQ_WAVES"):
         out["vector-memory reads per wave"] = v["SQ_INSTS_VMEM_RD"] / v["SQ_WAVES"]
+    if per_cu and "SQ_INSTS_VALU" in v:
+        # a wave64 VALU instruction (no packed math) issues over 4 cycles on
+        # one of a CU's 4 SIMDs: instructions / (CUs x cycles) is the share
+        # of SIMD cycles spent issuing VALU work
+        out["VALU issue share"] = v["SQ_INSTS_VALU"] / cus / per_cu
+    if per_cu and "SQ_LDS_IDX_ACTIVE" in v:
+        out["LDS active share"] = v["SQ_LDS_IDX_ACTIVE"] / cus / per_cu
+    if v.get("SQ_LDS_IDX_ACTIVE") and "SQ_LDS_BANK_CONFLICT" in v:
+        out["LDS bank-conflict share of LDS active"] = (v["SQ_LDS_BANK_CONFLICT"] /
+                                                        v["SQ_LDS_IDX_ACTIVE"])
+    if v.get("SQ_WAVE_CYCLES") and "SQ_WAIT_ANY" in v:
+        out["wave cycles waiting"] = v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"]
     return out
 
 
