@@ -621,3 +621,34 @@ def test_wide_dense_operands(name):
     m.spmm(d16, d_C, nv, btype=D.DSPMM_R_16BF, config=D.make_config(D.VARIANT_CSR_SCALAR, unroll=4))
     D.synchronize()
     assert_bitwise(d_C.numpy().reshape(-1), h32r, f"{name} nv={nv} bf16 row-seq")
+
+
+def test_split_combine_repeats_and_matches_batched():
+    """Split rows on float data (summation order matters): repeated calls,
+    calls covering nv in several column slabs and the SpMV tiles give the
+    same bits each time, within the FP32 gate, and the single-matrix
+    combine equals the batched launch's bit for bit (both sum a row's chunk
+    carries in one order fixed by nv and the chunk count)."""
+    c, m = matrix("zipf_mid")
+    assert (c.rowlen > 64).sum() > 0
+    for nv, cfg in ((128, D.make_config(D.VARIANT_NNZ_TILED, vec=4, lanes=16, tile_nnz=64)),
+                    (64, D.make_config(D.VARIANT_NNZ_TILED, vec=1, lanes=64, tile_nnz=16)),
+                    (1, D.make_config(D.VARIANT_SPMV_TILED, tile_nnz=512))):
+        what = f"zipf_mid split combine nv={nv} {cfg.as_dict()}"
+        note(what)
+        first = run_native(c, m, nv, config=cfg).copy()
+        for _ in range(3):
+            assert np.array_equal(bits(first), bits(run_native(c, m, nv, config=cfg))), what
+        h32, c64, absum = c.ref(RM, nv)
+        assert_fp32(first, c64, absum, c.rowlen_flat(RM, nv), what, h32)
+    # single launch against a one-item batch, the batch's static
+    # configuration on both
+    nv = 96
+    m.spmm(c.d_B, c.d_C, nv, config=D.make_config(D.VARIANT_NNZ_TILED))
+    D.synchronize()
+    single = c.output(nv).copy()
+    d_C2 = D.DeviceArray.zeros((c.nr, nv))
+    b = D.Batch([(m, nv, c.d_B, d_C2)])
+    b.run()
+    D.synchronize()
+    assert np.array_equal(bits(single), bits(d_C2.numpy().reshape(-1))), "single vs batched"
