@@ -374,7 +374,7 @@ def bench_batched(D, W, np, count, reps=20, sample_every=8):
                 worst[k] += f[k]
         return worst
 
-    for name, mode in (("batched", D.BATCH_PERSISTENT), ("graph", D.BATCH_GRAPH),
+    for name, mode in (("batched", D.BATCH_ONE_LAUNCH), ("graph", D.BATCH_GRAPH),
                        ("plain", D.BATCH_PLAIN), ("plain_own_best", D.BATCH_PLAIN_OWN)):
         for _, _, _, d_C in items:
             d_C.zero()
@@ -402,7 +402,7 @@ def bench_batched(D, W, np, count, reps=20, sample_every=8):
                  btype=D.DSPMM_R_16BF)
     for _, _, _, d_C in items:
         d_C.zero()
-    ms16, _ = bb.benchmark(D.BATCH_PERSISTENT, warmup=3, reps=reps)
+    ms16, _ = bb.benchmark(D.BATCH_ONE_LAUNCH, warmup=3, reps=reps)
     D.synchronize()
     res["batched_bf16_B"] = {"ms_mean": ms16, "req_per_s": count / (ms16 * 1e-3),
                              "GFLOP/s": flops_total / (ms16 * 1e-3) / 1e9,

@@ -282,6 +282,12 @@ dspmm_status_t dspmm_batch_describe(dspmm_batch_t b, dspmm_batch_info *info) {
     return DSPMM_STATUS_SUCCESS;
 }
 
+dspmm_status_t dspmm_batch_item_config(dspmm_batch_t b, int index, dspmm_config *out) {
+    if (!b || !out || index < 0 || index >= b->count) return DSPMM_STATUS_INVALID_VALUE;
+    *out = b->sch.cfg[(size_t)index];
+    return DSPMM_STATUS_SUCCESS;
+}
+
 dspmm_status_t dspmm_batch_run(dspmm_batch_t b, dspmm_stream_t stream) {
     if (!b) return DSPMM_STATUS_INVALID_VALUE;
     return run_batched(b, as_hip(stream), nullptr);

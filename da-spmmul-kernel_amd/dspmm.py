@@ -212,6 +212,7 @@ def _declare(lib):
         "dspmm_batch_destroy": ([_P], _ST),
         "dspmm_batch_describe": ([_P, C.POINTER(BatchInfo)], _ST),
         "dspmm_batch_run": ([_P, _P], _ST),
+        "dspmm_batch_item_config": ([_P, _I, C.POINTER(Config)], _ST),
         "dspmm_batch_graph_launch": ([_P, _P], _ST),
         "dspmm_batch_run_plain": ([_P, _P], _ST),
         "dspmm_batch_set_operands": ([_P, _I, _P, _P, _P], _ST),
@@ -542,11 +543,11 @@ def to_bf16_device(src: DeviceArray, stream=None) -> DeviceArray:
     return dst
 
 
-BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PLAIN, BATCH_PLAIN_OWN = 0, 1, 2, 3
+BATCH_ONE_LAUNCH, BATCH_GRAPH, BATCH_PLAIN, BATCH_PLAIN_OWN = 0, 1, 2, 3
 
 
 class Batch:
-    """Many row-major SpMMs in one persistent launch (dspmm_batch_*).
+    """Many row-major SpMMs in one launch (dspmm_batch_*).
 
     items: sequence of (Matrix, nv, B, C) or (Matrix, nv, B, C, ldb, ldc)."""
 
@@ -576,6 +577,12 @@ class Batch:
     def run(self, stream=None):
         check(lib().dspmm_batch_run(self.handle, stream), "batch_run")
 
+    def item_config(self, index) -> "Config":
+        """The configuration item `index` runs with in the batch."""
+        out = Config()
+        check(lib().dspmm_batch_item_config(self.handle, index, C.byref(out)), "batch_item_config")
+        return out
+
     def graph_launch(self, stream=None):
         check(lib().dspmm_batch_graph_launch(self.handle, stream), "batch_graph_launch")
 
@@ -586,7 +593,7 @@ class Batch:
         check(lib().dspmm_batch_set_operands(self.handle, index, ptr(B), ptr(C_out), stream),
               "batch_set_operands")
 
-    def benchmark(self, mode=BATCH_PERSISTENT, warmup=3, reps=20, stream=None):
+    def benchmark(self, mode=BATCH_ONE_LAUNCH, warmup=3, reps=20, stream=None):
         mean = C.c_float()
         arr = (C.c_float * reps)()
         check(lib().dspmm_benchmark_batch(self.handle, mode, warmup, reps, C.byref(mean), arr,

@@ -72,12 +72,41 @@ def last_dispatches(trace_csv, prefix, n):
             "avg_ns": sum(d for _, d, _ in tail) / len(tail)}
 
 
+def passes_after_copy(trace_csv, passes):
+    """The dispatches after the last runtime copy (the calibration copy that
+    precedes the timed passes), cut into `passes` equal runs: per run, the
+    dispatch count, the wall time from first start to last end, and the sum
+    of the kernels' own durations."""
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    copies = [i for i, r in enumerate(rows) if "copyBuffer" in r[2]]
+    after = rows[copies[-1] + 1:] if copies else rows
+    n = len(after) // passes
+    out = []
+    for p in range(passes):
+        seg = after[p * n:(p + 1) * n]
+        if seg:
+            out.append({"dispatches": len(seg), "wall_us": (seg[-1][1] - seg[0][0]) / 1e3,
+                        "kernel_us": sum(e - b for b, e, _ in seg) / 1e3})
+    return out
+
+
 def trace_only(args):
     stats = kernel_stats(args.trace)
     rows = sorted(stats.items(), key=lambda kv: -kv[1]["avg_ns"] * kv[1]["calls"])
     with open(args.out, "w") as f:
         f.write(f"# rocprofv3 kernel trace: {args.title}\n\n")
         f.write(f"Command: `{args.command}`\n\n")
+        if args.passes:
+            trace_csv = args.trace.replace("kernel_stats.csv", "kernel_trace.csv")
+            f.write("Timed passes (the dispatches after the calibration copy, in equal runs):\n\n")
+            f.write("| pass | dispatches | wall us | summed kernel us |\n|---|---|---|---|\n")
+            for i, p in enumerate(passes_after_copy(trace_csv, args.passes)):
+                f.write(f"| {i} | {p['dispatches']} | {p['wall_us']:.1f} | {p['kernel_us']:.1f} |\n")
+            f.write("\n")
         if args.last and args.kernel:
             trace_csv = args.trace.replace("kernel_stats.csv", "kernel_trace.csv")
             t = last_dispatches(trace_csv, args.kernel, args.last)
@@ -178,6 +207,9 @@ def main():
     ap.add_argument("--kernel", default="", help="trace-only mode: kernel name prefix whose "
                     "last --last dispatches are averaged (the timed steps)")
     ap.add_argument("--last", type=int, default=0)
+    ap.add_argument("--passes", type=int, default=0,
+                    help="trace-only mode: cut the dispatches after the calibration copy into "
+                         "this many timed passes")
     args = ap.parse_args()
     if args.pmc:
         return pmc_mode(args)

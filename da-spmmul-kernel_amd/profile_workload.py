@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--unroll", type=int, default=0)
     ap.add_argument("--row-stride", type=int, default=0)
+    ap.add_argument("--batch-mode", default="one", choices=["one", "plain_own"],
+                    help="c4: the batch's one launch, or one launch per matrix with each "
+                         "matrix's own autotuned kernel (the matrices are autotuned first)")
     args = ap.parse_args()
 
     import numpy as np
@@ -93,16 +96,23 @@ def batch_c4(args, np, D, W, count=512):
                       D.DeviceArray.zeros((a.nr, nv))))
         nnz += a.nnz
         alg += W.algorithmic_bytes(a.nr, a.nc, a.nnz, nv)
+    if args.batch_mode == "plain_own":
+        for mat, nv, d_B, d_C in items:
+            mat.autotune(d_B, d_C, nv, reps=2)
     b = D.Batch(items)
     src = D.DeviceArray(COPY_BYTES, np.uint8)
     dst = D.DeviceArray(COPY_BYTES, np.uint8)
     D.check(D.lib().dspmm_memcpy_d2d(dst.ptr, src.ptr, COPY_BYTES, None), "d2d")
     D.synchronize()
     for _ in range(args.steps):
-        b.run()
+        if args.batch_mode == "plain_own":
+            b.benchmark(D.BATCH_PLAIN_OWN, warmup=0, reps=1)
+        else:
+            b.run()
     D.synchronize()
     print(json.dumps({"workload": "c4", "n": 0, "bf16": False, "nnz": nnz,
-                      "config": {"variant": "batched", "matrices": count},
+                      "config": {"variant": "batched" if args.batch_mode == "one" else "plain_own_best",
+                                 "matrices": count},
                       "copy_bytes": COPY_BYTES, "algorithmic_bytes": alg}))
 
 

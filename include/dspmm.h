@@ -415,14 +415,17 @@ DSPMM_API dspmm_status_t dspmm_convert_f32_to_bf16(const float *src, void *dst, 
 /* ---------------------------------------------------------------------
  * Batched SpMM (no reference counterpart): many independent row-major
  * products C_i = A_i * B_i, each with its own matrix, nv and operands,
- * in ONE persistent launch (plus one launch that combines split rows).
- * Each item runs the nnz-tiled kernel with the static default vector
- * width, lanes and tile for its nv and dense type (autotuned single-launch
- * configurations are not used); results are bit-identical to dspmm_spmm_ex
- * with that configuration (DSPMM_VARIANT_NNZ_TILED, other fields 0).  Work is
- * cut into wavefront units; each XCD owns a list of them, and a matrix is
- * kept on one XCD when it is small enough for its B to stay in that XCD's
- * L2.  The matrices must outlive the batch.
+ * in ONE launch (plus one launch that combines split rows).  Each item
+ * runs the nnz-tiled kernel with the static default vector width, lanes and
+ * tile for its nv and dense type (autotuned single-launch configurations
+ * are not used); dspmm_batch_item_config returns it, and dspmm_spmm_ex with
+ * it gives the same bits.  Work is cut into wavefront units, slab-major
+ * within an item; each XCD owns a list of them and walks it in order, and a
+ * matrix is kept on one XCD unless it is heavy (then its units are cut into
+ * eight contiguous ranges, one per XCD), so that a matrix whose B fits in
+ * an XCD's L2 is gathered from there.  The default grid is one workgroup
+ * per four units; blocks_per_cu > 0 gives a persistent grid.  The matrices
+ * must outlive the batch.
  * ------------------------------------------------------------------- */
 typedef struct dspmm_batch_s *dspmm_batch_t;
 
@@ -440,7 +443,7 @@ typedef struct dspmm_batch_info {
     int units;           /* wavefront work units */
     int segments;
     int splits;          /* split rows combined after the main launch */
-    int blocks;          /* persistent workgroups */
+    int blocks;          /* workgroups of the one launch */
     int units_per_xcd[8];
     size_t table_bytes;  /* device bytes of the schedule tables */
     size_t carry_bytes;  /* device bytes of split-row partial sums */
@@ -453,8 +456,14 @@ DSPMM_API dspmm_status_t dspmm_batch_create(dspmm_batch_t *out, const dspmm_batc
                                             int count, dspmm_dtype_t btype, int blocks_per_cu);
 DSPMM_API dspmm_status_t dspmm_batch_destroy(dspmm_batch_t b);
 DSPMM_API dspmm_status_t dspmm_batch_describe(dspmm_batch_t b, dspmm_batch_info *info);
-/* Persistent launch (+ split combine) on `stream`. */
+/* The batch's one launch (+ split combine) on `stream`. */
 DSPMM_API dspmm_status_t dspmm_batch_run(dspmm_batch_t b, dspmm_stream_t stream);
+/* The configuration item `index` runs with in the batch (the nnz-tiled
+ * kernel's static default); a single dspmm_spmm_ex call with it gives the
+ * same bits.
+ * No counterpart in the reference; for callers that check or replay one
+ * item alone. */
+DSPMM_API dspmm_status_t dspmm_batch_item_config(dspmm_batch_t b, int index, dspmm_config *out);
 /* Replays a hipGraph of dspmm_batch_run (captured on first call). */
 DSPMM_API dspmm_status_t dspmm_batch_graph_launch(dspmm_batch_t b, dspmm_stream_t stream);
 /* Baseline: one dspmm_spmm_ex launch per item, same configs, in order. */

@@ -1,4 +1,4 @@
-"""GPU tests of the batched path: one persistent launch over many matrices
+"""GPU tests of the batched path: one launch over many matrices
 equals the single-matrix kernel bit for bit (same per-matrix config), is
 within the FP32 gate of the FP64 oracle, and its hipGraph replay follows
 operand rebinding (set_operands) across replays with eager work between
@@ -74,11 +74,10 @@ def test_batched_equals_single_and_oracle():
     note(f"batch: {info}")
     assert info["units"] > 0 and info["blocks"] % 8 == 0
     b.run()
-    for it in items:
-        # single-matrix path with the config the batch resolved: an untuned
-        # matrix's static nnz-tiled default (AUTO on a single launch may pick
-        # another kernel or tile)
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=D.make_config(D.VARIANT_NNZ_TILED))
+    for i, it in enumerate(items):
+        # single-matrix path with the configuration the batch runs the item
+        # with (AUTO on a single launch may pick another kernel or tile)
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=b.item_config(i))
     D.synchronize()
     for i, it in enumerate(items):
         out, ref = it.d_C.numpy().reshape(-1), it.d_C_ref.numpy().reshape(-1)
@@ -131,8 +130,7 @@ def test_batched_bf16():
     b.run()
     D.synchronize()
     for i, it in enumerate(items):
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv, btype=D.DSPMM_R_16BF,
-                  config=D.make_config(D.VARIANT_NNZ_TILED))
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, btype=D.DSPMM_R_16BF, config=b.item_config(i))
     D.synchronize()
     for i, it in enumerate(items):
         out = it.d_C.numpy().reshape(-1)
@@ -144,7 +142,7 @@ def test_batched_bf16():
 def test_benchmark_modes_run():
     items = make_items(16, seed=44)
     b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
-    for mode in (D.BATCH_PERSISTENT, D.BATCH_GRAPH, D.BATCH_PLAIN, D.BATCH_PLAIN_OWN):
+    for mode in (D.BATCH_ONE_LAUNCH, D.BATCH_GRAPH, D.BATCH_PLAIN, D.BATCH_PLAIN_OWN):
         ms, per = b.benchmark(mode, warmup=1, reps=3)
         note(f"mode {mode}: {ms * 1e3:.1f} us")
         assert ms > 0 and len(per) == 3
@@ -169,8 +167,8 @@ def test_batch_with_empty_and_tiny_matrices():
     b = D.Batch([(it.m, it.nv, it.d_B, it.d_C) for it in items])
     b.run()
     D.synchronize()
-    for it in items:
-        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=D.make_config(D.VARIANT_NNZ_TILED))
+    for i, it in enumerate(items):
+        it.m.spmm(it.d_B, it.d_C_ref, it.nv, config=b.item_config(i))
     D.synchronize()
     for i, it in enumerate(items):
         if it.a.nr == 0:

@@ -641,14 +641,14 @@ def test_split_combine_repeats_and_matches_batched():
             assert np.array_equal(bits(first), bits(run_native(c, m, nv, config=cfg))), what
         h32, c64, absum = c.ref(RM, nv)
         assert_fp32(first, c64, absum, c.rowlen_flat(RM, nv), what, h32)
-    # single launch against a one-item batch, the batch's static
-    # configuration on both
+    # single launch against a one-item batch, the batch's configuration
+    # for the item on both
     nv = 96
-    m.spmm(c.d_B, c.d_C, nv, config=D.make_config(D.VARIANT_NNZ_TILED))
-    D.synchronize()
-    single = c.output(nv).copy()
     d_C2 = D.DeviceArray.zeros((c.nr, nv))
     b = D.Batch([(m, nv, c.d_B, d_C2)])
     b.run()
     D.synchronize()
+    m.spmm(c.d_B, c.d_C, nv, config=b.item_config(0))
+    D.synchronize()
+    single = c.output(nv).copy()
     assert np.array_equal(bits(single), bits(d_C2.numpy().reshape(-1))), "single vs batched"
