@@ -15,9 +15,65 @@
 
 namespace dspmm {
 
+dspmm_status_t schedule_item(dspmm_matrix_s *A, int nv, int64_t ldb, int64_t ldc, const void *B,
+                             const void *C, dspmm_dtype_t btype, ItemFragment &f) {
+    const bool bf16 = btype == DSPMM_R_16BF;
+    f = ItemFragment{};
+    if (!A || nv < 1 || ldb < nv || ldc < nv) return DSPMM_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> lk(A->mu);
+    // The static nnz-tiled default: configurations tuned on single
+    // launches (where one matrix has the whole GPU and its caches) make
+    // the batch slower, not faster.
+    dspmm_config c{};
+    c.variant = DSPMM_VARIANT_NNZ_TILED;
+    complete_nb(c, nv, ldb, ldc, B, C, bf16);
+    if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) {
+        c.tile_nnz = 16;
+        if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) return DSPMM_STATUS_NOT_SUPPORTED;
+    }
+    NbPlan *plan = nullptr;
+    dspmm_status_t st = get_plan(A, c.tile_nnz, &plan);
+    if (st != DSPMM_STATUS_SUCCESS) return st;
+    f.cfg = c;
+    ItemDesc &d = f.d;
+    d.tiles = plan->d_tiles;
+    d.rowPtr = A->rowPtr;
+    d.colIdx = A->colIdx;
+    d.values = A->values;
+    d.ldcarry = carry_ld(nv);
+    d.nv = nv;
+    d.slab_w = c.vec * c.lanes;
+    d.S = c.tile_nnz;
+    d.cls = item_class(c.vec, c.lanes);
+    d.ntiles = plan->ntiles;
+    d.nslabs = div_up(nv, d.slab_w);
+    d.splits = plan->d_splits;
+    d.nsplits = plan->nsplits;
+    const int G = kWave / c.lanes;
+    f.units = d.nslabs * div_up(plan->ntiles, G);
+    f.cost = (double)A->nnz * (8.0 + (double)nv * (bf16 ? 2 : 4)) + (double)A->nr * nv * 4.0;
+    f.lds_ints = (int)(nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) / (kBlock / kWave) / 4);
+    f.nslots = plan->nslots;
+    f.nsplits = plan->nsplits;
+    return DSPMM_STATUS_SUCCESS;
+}
+
 dspmm_status_t build_schedule(const dspmm_batch_item *items, int count, dspmm_dtype_t btype,
                               BatchSchedule &s) {
-    const bool bf16 = btype == DSPMM_R_16BF;
+    std::vector<ItemFragment> frags((size_t)count);
+    std::vector<const ItemFragment *> fp((size_t)count);
+    for (int i = 0; i < count; i++) {
+        const dspmm_batch_item &it = items[i];
+        if (!it.A || (it.A->nr > 0 && !it.C) || (it.A->nnz > 0 && !it.B)) return DSPMM_STATUS_INVALID_VALUE;
+        dspmm_status_t st = schedule_item(it.A, it.nv, it.ldb, it.ldc, it.B, it.C, btype, frags[(size_t)i]);
+        if (st != DSPMM_STATUS_SUCCESS) return st;
+        fp[(size_t)i] = &frags[(size_t)i];
+    }
+    return assemble_schedule(fp.data(), items, count, s);
+}
+
+dspmm_status_t assemble_schedule(const ItemFragment *const *frags, const dspmm_batch_item *items,
+                                 int count, BatchSchedule &s) {
     s = BatchSchedule{};
     s.cfg.resize((size_t)count);
     s.items.resize((size_t)count);
@@ -27,55 +83,23 @@ dspmm_status_t build_schedule(const dspmm_batch_item *items, int count, dspmm_dt
     std::vector<int> units((size_t)count, 0);
     std::vector<double> cost((size_t)count, 0.0);
     for (int i = 0; i < count; i++) {
+        const ItemFragment &f = *frags[i];
         const dspmm_batch_item &it = items[i];
-        dspmm_matrix_s *A = it.A;
-        if (!A || it.nv < 1 || it.ldb < it.nv || it.ldc < it.nv || (A->nr > 0 && !it.C) ||
-            (A->nnz > 0 && !it.B))
-            return DSPMM_STATUS_INVALID_VALUE;
-        std::lock_guard<std::mutex> lk(A->mu);
-        // The static nnz-tiled default: configurations tuned on single
-        // launches (where one matrix has the whole GPU and its caches) make
-        // the batch slower, not faster.
-        dspmm_config c{};
-        c.variant = DSPMM_VARIANT_NNZ_TILED;
-        complete_nb(c, it.nv, it.ldb, it.ldc, it.B, it.C, bf16);
-        if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) {
-            c.tile_nnz = 16;
-            if (nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) > kMaxTileLds) return DSPMM_STATUS_NOT_SUPPORTED;
-        }
-        NbPlan *plan = nullptr;
-        dspmm_status_t st = get_plan(A, c.tile_nnz, &plan);
-        if (st != DSPMM_STATUS_SUCCESS) return st;
-        s.cfg[(size_t)i] = c;
+        s.cfg[(size_t)i] = f.cfg;
         ItemDesc &d = s.items[(size_t)i];
-        d.tiles = plan->d_tiles;
-        d.rowPtr = A->rowPtr;
-        d.colIdx = A->colIdx;
-        d.values = A->values;
+        d = f.d;
         d.B = it.B;
         d.C = it.C;
         d.carry = nullptr;
         d.ldb = it.ldb;
         d.ldc = it.ldc;
-        d.ldcarry = carry_ld(it.nv);
-        d.nv = it.nv;
-        d.slab_w = c.vec * c.lanes;
-        d.S = c.tile_nnz;
-        d.cls = item_class(c.vec, c.lanes);
-        d.ntiles = plan->ntiles;
-        d.nslabs = div_up(it.nv, d.slab_w);
-        d.splits = plan->d_splits;
-        d.nsplits = plan->nsplits;
-        const int G = kWave / c.lanes;
-        units[(size_t)i] = d.nslabs * div_up(plan->ntiles, G);
-        cost[(size_t)i] = (double)A->nnz * (8.0 + (double)it.nv * (bf16 ? 2 : 4)) +
-                          (double)A->nr * it.nv * 4.0;
-        s.lds_ints = std::max(s.lds_ints,
-                              (int)(nnz_tiled_lds_bytes(c.lanes, c.tile_nnz) / (kBlock / kWave) / 4));
+        units[(size_t)i] = f.units;
+        cost[(size_t)i] = f.cost;
+        s.lds_ints = std::max(s.lds_ints, f.lds_ints);
         s.carry_off[(size_t)i] = s.carry_floats;
-        s.carry_bytes[(size_t)i] = sizeof(float) * (size_t)plan->nslots * (size_t)d.ldcarry;
-        s.carry_floats += (int64_t)plan->nslots * d.ldcarry;
-        s.split_prefix[(size_t)i + 1] = s.split_prefix[(size_t)i] + plan->nsplits;
+        s.carry_bytes[(size_t)i] = sizeof(float) * (size_t)f.nslots * (size_t)d.ldcarry;
+        s.carry_floats += (int64_t)f.nslots * d.ldcarry;
+        s.split_prefix[(size_t)i + 1] = s.split_prefix[(size_t)i] + f.nsplits;
     }
     s.nsplits = s.split_prefix[(size_t)count];
 

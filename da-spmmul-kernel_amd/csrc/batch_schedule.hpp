@@ -23,9 +23,29 @@ struct BatchSchedule {
     int units_x[8] = {0};
 };
 
-// Resolves each item's config (its matrix's autotuned or heuristic
-// nnz-tiled config), fetches the tile plans and cuts the work into
-// per-XCD segments.  Locks each matrix while reading it.
+// One item's part of a schedule: its nnz-tiled configuration, descriptor
+// (operands, strides and carry left to bind), work units, cost and carry
+// size.  It depends on the matrix, nv, the dense type and the operands'
+// alignment class only, so a server computes it once per (matrix, nv) and
+// every later admission copies it.
+struct ItemFragment {
+    dspmm_config cfg{};
+    ItemDesc d{};
+    int units = 0;
+    double cost = 0;
+    int lds_ints = 1;
+    int nslots = 0, nsplits = 0;
+};
+
+// The fragment of matrix A with nv columns (locks A while reading it).
+dspmm_status_t schedule_item(dspmm_matrix_s *A, int nv, int64_t ldb, int64_t ldc, const void *B,
+                             const void *C, dspmm_dtype_t btype, ItemFragment &f);
+// A schedule from one fragment per item (frags[i] for items[i], whose
+// operands and strides are bound here): carry offsets, split prefix sums and
+// the per-XCD segments.
+dspmm_status_t assemble_schedule(const ItemFragment *const *frags, const dspmm_batch_item *items,
+                                 int count, BatchSchedule &s);
+// schedule_item for every item, then assemble_schedule.
 dspmm_status_t build_schedule(const dspmm_batch_item *items, int count, dspmm_dtype_t btype,
                               BatchSchedule &s);
 // Workgroups of the launch: blocks_per_cu > 0 -> persistent grid.

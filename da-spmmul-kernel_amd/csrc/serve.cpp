@@ -41,6 +41,11 @@ size_t round_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
 // Matrices with at least this many nonzeros fill the GPU alone; the batching
 // server launches their requests one at a time.
 constexpr int64_t kServeAloneNnz = 16 << 20;
+// Up to this many admitted requests whose own AUTO kernel is row-exact (and
+// whose batch plan splits no row) launch one by one with it instead of as
+// one batched launch: few requests gain nothing from the batch's static
+// kernel, and launch one after another on the slot's stream.
+constexpr size_t kServeOwnMax = 4;
 
 // First-fit allocator over one device region; offsets aligned to 256 B.
 class Arena {
@@ -97,6 +102,18 @@ class Arena {
     std::map<size_t, size_t> free_;
 };
 
+// Per (matrix, nv): the matrix's part of a batch schedule, and whether a
+// request that finds no company may run the matrix's own AUTO kernel
+// instead -- only where that kernel sums every row in nonzero order, as the
+// batch's kernel does when it splits no row, so the result has the same
+// bits either way.
+struct Frag {
+    ItemFragment f;
+    dspmm_config own{};
+    size_t own_ws = 0;
+    bool own_ok = false;
+};
+
 struct SMat {
     int nr = 0, nc = 0, nnz = 0;
     std::vector<int> rp, ci;
@@ -105,6 +122,7 @@ struct SMat {
     size_t bytes = 0;
     uint64_t last_use = 0;
     int pins = 0;
+    std::map<int, Frag> frags;         // by nv; cleared when the handle goes
     size_t estimate() const {
         return 4 * ((size_t)nr + 1) + 8 * (size_t)nnz + ((size_t)nr + (size_t)nnz) + (1 << 20);
     }
@@ -174,6 +192,7 @@ struct dspmm_server_s {
             // the device), so submit / poll callers are never held up.
             evicted.push_back(victim->h);
             victim->h = nullptr;
+            victim->frags.clear();
             cache_used -= victim->bytes;
             victim->bytes = 0;
             stats.evictions++;
@@ -200,6 +219,46 @@ struct dspmm_server_s {
         return cfg.plain == 0 && (int64_t)m.nnz >= (cfg.alone_nnz > 0 ? cfg.alone_nnz : kServeAloneNnz);
     }
 
+    // The (matrix, nv) fragment, computed on the first request and kept
+    // while the matrix stays resident: admission copies it.  The arena's
+    // operands are 256-byte aligned with ld = nv, so one alignment class.
+    Frag *fragment(SMat &m, int nv) {
+        auto it = m.frags.find(nv);
+        if (it != m.frags.end()) return &it->second;
+        Frag fr;
+        if (schedule_item(m.h, nv, nv, nv, reinterpret_cast<const void *>(256),
+                          reinterpret_cast<const void *>(256), cfg.btype, fr.f) != DSPMM_STATUS_SUCCESS)
+            return nullptr;
+        if (dspmm_spmm_ex_resolve(m.h, DENSE_ROW_MAJOR, nv, cfg.btype, nullptr, &fr.own) ==
+                DSPMM_STATUS_SUCCESS &&
+            dspmm_spmm_ex_workspace_size(m.h, DENSE_ROW_MAJOR, nv, cfg.btype, &fr.own, &fr.own_ws) ==
+                DSPMM_STATUS_SUCCESS)
+            fr.own_ok = fr.f.nsplits == 0 && row_exact(m.h, nv, fr.own);
+        return &m.frags.emplace(nv, fr).first->second;
+    }
+
+    // Kernels that sum every row in nonzero order with a separately rounded
+    // multiply and add (the nnz-tiled kernel when its plan splits no row).
+    bool row_exact(dspmm_matrix_t h, int nv, const dspmm_config &c) const {
+        switch (c.variant) {
+        case DSPMM_VARIANT_CSR_SCALAR:
+        case DSPMM_VARIANT_NNZ_CACHED:
+        case DSPMM_VARIANT_ROW_BLOCKED:
+        case DSPMM_VARIANT_RING:
+            return true;
+        case DSPMM_VARIANT_NNZ_TILED: {
+            dspmm_config cc = c;
+            complete_nb(cc, nv, nv, nv, reinterpret_cast<const void *>(256),
+                        reinterpret_cast<const void *>(256), cfg.btype == DSPMM_R_16BF);
+            std::lock_guard<std::mutex> lk(h->mu);
+            NbPlan *plan = nullptr;
+            return get_plan(h, cc.tile_nnz, &plan) == DSPMM_STATUS_SUCCESS && plan->nsplits == 0;
+        }
+        default:
+            return false;
+        }
+    }
+
     bool reserve(Job &j, SMat &m) {
         const dspmm_request &r = j.r;
         j.szB = r.B ? 0 : (size_t)m.nc * (size_t)r.nv * elem_b();
@@ -210,12 +269,9 @@ struct dspmm_server_s {
             // so the handle's internal buffer cannot be shared)
             (void)dspmm_spmm_ex_workspace_size(m.h, DENSE_ROW_MAJOR, r.nv, cfg.btype, nullptr,
                                                &carry);
-        } else {
-            dspmm_batch_item it{m.h, r.nv, reinterpret_cast<const void *>(256), r.nv,
-                                reinterpret_cast<float *>(256), r.nv};
-            BatchSchedule one;
-            if (build_schedule(&it, 1, cfg.btype, one) == DSPMM_STATUS_SUCCESS)
-                carry = one.carry_bytes[0];
+        } else if (const Frag *fr = fragment(m, r.nv)) {
+            carry = sizeof(float) * (size_t)fr->f.nslots * (size_t)fr->f.d.ldcarry;
+            if (fr->own_ok) carry = std::max(carry, fr->own_ws);
         }
         j.szK = carry;
         bool okB = true, okC = true, okK = true;
@@ -285,10 +341,38 @@ struct dspmm_server_s {
                                     cfg.btype == DSPMM_R_16BF, sl.st);
         }
         BatchSchedule sch;
-        dspmm_status_t st = build_schedule(items.data(), (int)items.size(), cfg.btype, sch);
+        std::vector<const ItemFragment *> fp;
+        std::vector<const Frag *> own;                 // per job, while all may run their own kernel
+        dspmm_status_t st = DSPMM_STATUS_SUCCESS;
+        if (!alone && !cfg.plain) {
+            for (auto &j : jobs) {
+                const Frag *fr = fragment(mats[(size_t)j.r.matrix], j.r.nv);
+                if (!fr) {
+                    st = DSPMM_STATUS_NOT_SUPPORTED;
+                    break;
+                }
+                fp.push_back(&fr->f);
+                if (fr->own_ok && fr->own_ws <= j.szK) own.push_back(fr);
+            }
+            if (own.size() != jobs.size() || jobs.size() > kServeOwnMax) own.clear();
+            if (st == DSPMM_STATUS_SUCCESS && own.empty())
+                st = assemble_schedule(fp.data(), items.data(), (int)items.size(), sch);
+        } else {
+            st = build_schedule(items.data(), (int)items.size(), cfg.btype, sch);
+        }
         if (st == DSPMM_STATUS_SUCCESS) {
-            for (size_t i = 0; i < jobs.size(); i++) sch.items[i].carry = jobs[i].carry;
-            if (alone) {
+            for (size_t i = 0; i < jobs.size() && i < sch.items.size(); i++) sch.items[i].carry = jobs[i].carry;
+            if (!own.empty()) {
+                // a few requests, each on a row-exact kernel of its own
+                // matrix: one launch each on the slot's stream (the same bits
+                // as the batched launch; at low load, lower latency)
+                for (size_t i = 0; i < jobs.size() && st == DSPMM_STATUS_SUCCESS; i++) {
+                    const Job &j = jobs[i];
+                    st = dspmm_spmm_ex(items[i].A, DENSE_ROW_MAJOR, j.r.nv, j.B, j.r.nv, cfg.btype,
+                                       j.C, j.r.nv, &own[i]->own, j.carry, j.szK,
+                                       reinterpret_cast<dspmm_stream_t>(sl.st));
+                }
+            } else if (alone) {
                 const Job &j = jobs[0];
                 st = dspmm_spmm_ex(items[0].A, DENSE_ROW_MAJOR, j.r.nv, j.B, j.r.nv, cfg.btype, j.C,
                                    j.r.nv, nullptr, j.carry, j.szK,

@@ -490,8 +490,14 @@ DSPMM_API dspmm_status_t dspmm_benchmark_batch(dspmm_batch_t b, int mode, int wa
  * max_inflight stream slots is free, every admissible queued request (up
  * to max_batch) is launched as one batched SpMM on that slot, so batches
  * overlap on the device and requests that arrive meanwhile join the next
- * one.  Results are bit-identical to running the request alone (the
- * kernels are deterministic and each matrix keeps its config).  In plain
+ * one.  A matrix's part of a batch schedule is computed on its first
+ * request with a given nv and kept while it stays resident, so admission
+ * copies it.  Up to four admitted requests whose matrix's own AUTO kernel
+ * sums each row in nonzero order (and whose batch plan splits no row)
+ * launch one by one with that kernel instead, which at low load is faster
+ * than the batch's static kernel and gives the same bits.  Results are
+ * bit-identical to running the request alone (the kernels are
+ * deterministic and each matrix keeps its config).  In plain
  * mode every request is one dspmm_spmm_ex launch on a single stream (the
  * baseline).  Times are seconds on dspmm_server_now()'s steady clock.
  * ------------------------------------------------------------------- */

@@ -155,3 +155,66 @@ def test_batch_of_512_equals_plain_launches():
     b.close()
     for it in items:
         it[0].close()
+
+
+@pytest.mark.parametrize("btype", [D.DSPMM_R_32F, D.DSPMM_R_16BF])
+def test_server_serves_the_large_matrix_alone_and_in_a_mix(btype):
+    """The serving loop on the large power-law matrix (M = K = 2e6, ~40 M
+    nonzeros, N = 64; it runs alone, with its own AUTO kernel and a
+    workspace of its own): once with the server otherwise idle, once in a
+    burst among small batched requests.  Both give the same bits (same
+    seed), and sampled rows are within the FP32 gate of the FP64 product on
+    the device-generated B (rounded to BF16 for BF16 storage); the small
+    requests of the mix equal the same requests served alone."""
+    a = W.c5_large_matrix(seed=500)
+    nv = 64
+    small = [(W.gen_csr(W.GEN_BINOMIAL, 3000, 3000, 7100 + k, p0=12.0), 32) for k in range(6)]
+    pool = [(a, nv)] + small
+    srv = D.Server(cache_bytes=8 << 30, operand_bytes=8 << 30, max_inflight=3, max_batch=64,
+                   btype=btype)
+    mids = [srv.add_matrix(x.nr, x.nc, x.rowPtr, x.colIdx, x.values, preload=True) for x, _ in pool]
+
+    def served(comps, of):
+        out = {}
+        for c in comps:
+            assert c["status"] == 0, c
+            x, n = pool[of[c["id"]]]
+            out[c["id"]] = D.device_read(c["C"], (x.nr * n,)).copy()
+            srv.release(c["id"])
+        return out
+
+    note(f"large alone (btype {btype})")
+    assert srv.submit(srv.request(1, mids[0], nv, 4242)) == 0
+    srv.drain()
+    comps = srv.poll()
+    assert len(comps) == 1 and comps[0]["batch_size"] == 1
+    alone = served(comps, {1: 0})[1]
+    lone_small = {}
+    for k in range(len(small)):
+        assert srv.submit(srv.request(100 + k, mids[1 + k], 32, 600 + k)) == 0
+        srv.drain()
+        lone_small.update(served(srv.poll(), {100 + k: 1 + k}))
+    note("large in a mix")
+    reqs, of = [], {}
+    for k in range(40):
+        mi = 1 + k % len(small)
+        reqs.append(srv.request(1000 + k, mids[mi], 32, 600 + (mi - 1)))
+        of[1000 + k] = mi
+        if k == 20:
+            reqs.append(srv.request(2, mids[0], nv, 4242))
+            of[2] = 0
+    srv.replay(reqs, [0.0] * len(reqs))
+    comps = srv.poll()
+    assert len(comps) == len(reqs)
+    assert max(c["batch_size"] for c in comps) > 1
+    got = served(comps, of)
+    assert np.array_equal(bits(got[2]), bits(alone)), "large request: mix vs alone"
+    for rid, mi in of.items():
+        if mi > 0:
+            assert np.array_equal(bits(got[rid]), bits(lone_small[100 + mi - 1])), f"small {rid}"
+    B = D.seeded_uniform(4242, a.nc * nv)
+    if btype == D.DSPMM_R_16BF:
+        B = O.round_bf16(B)
+    check_sampled_rows(a, sample(a, 2000, 52), B, nv, alone, f"served large btype {btype}",
+                       exact=False)
+    srv.close()
