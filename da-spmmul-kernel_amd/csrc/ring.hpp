@@ -26,7 +26,10 @@ struct RingShape {
 
 // Host plan (see ring_plan.cpp).
 struct RingPlanHost {
-    std::vector<int4> tiles;          // {p_begin, p_end, q_begin, q_end} (q: index into row_info)
+    // {p_begin, longest row, q_begin, q_end} (q: index into row_info).  The tile's
+    // nonzeros are p_begin + [0, rows x padded length), the padded length being
+    // the longest row rounded up to 4, then rounded up to 8 entries.
+    std::vector<int4> tiles;
     std::vector<int> stream_off;      // nstreams + 1 offsets into tiles
     std::vector<int2> row_info;       // {row, end of its padded nonzeros relative to p_begin}
     std::vector<int> fill_off;        // ntiles + 1 offsets into fills

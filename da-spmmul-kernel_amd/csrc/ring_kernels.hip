@@ -190,7 +190,7 @@ struct MetaRegs {
     f32x4_t v0, v1;
     piece_t sl;
     int2 row;
-    int nch, nrows;
+    int nch, nrows, lmax;
 };
 
 __device__ __forceinline__ void meta_load(const RingLaunch &a, int4 td, MetaRegs &m) {
@@ -200,8 +200,11 @@ __device__ __forceinline__ void meta_load(const RingLaunch &a, int4 td, MetaRegs
     td.y = __builtin_amdgcn_readfirstlane(td.y);
     td.z = __builtin_amdgcn_readfirstlane(td.z);
     td.w = __builtin_amdgcn_readfirstlane(td.w);
-    m.nch = (td.y - td.x) >> 3;
+    // td.y: the tile's longest row; its padded entries are rows x that
+    // length rounded up to 4, then to 8
     m.nrows = td.w - td.z;
+    m.lmax = td.y;
+    m.nch = (m.nrows * ((td.y + 3) & ~3) + 7) >> 3;
     const int w0 = __builtin_amdgcn_readfirstlane(t) & ~63;   // the wave's first thread
     if (w0 < m.nch) {
         const int c = min(t, m.nch - 1);
@@ -236,18 +239,54 @@ __device__ __forceinline__ void meta_store(const MetaBuf &s, const MetaRegs &m, 
 // those again, the same bits to the same place: every lane issues the same
 // stores, with no branch, so the count of memory operations in flight
 // after them is known and a later wait for loads does not wait for them.
-// All rows of a tile are padded to one length, a multiple of 4 (zero-slot
-// entries), so the loop over a row's entries, four at a time (offsets and
-// values in 16-byte reads), has no bounds inside.
+// All rows of a tile are padded to one length, the longest row rounded up
+// to 4 (zero-slot entries), so the loop over a row's entries, four at a
+// time (offsets and values in 16-byte reads), has no bounds inside; it ends
+// at the longest row, so at most the shorter rows' padding is summed.
 template <int VW, int R>
 struct TileRows {
     float acc[R][VW];
     uint32_t dst[R];         // byte offset in C of the row's VW columns
 };
 
+// One step of four entries of every row the group holds (n of them summed:
+// the rest of a row's last step past the tile's longest row are padding,
+// whose +0 products are skipped -- the same sums).
+template <typename TB, int VW, int R, int N>
+__device__ __forceinline__ void ring_step(const unsigned char *rbase, const MetaBuf &s,
+                                          const int (&base)[R], int p, TileRows<VW, R> &out) {
+    int4 o[R];
+    float4 v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        o[j] = *reinterpret_cast<const int4 *>(s.offs + base[j] + p);
+        v[j] = *reinterpret_cast<const float4 *>(s.vals + base[j] + p);
+    }
+    float b[R][4][VW];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        load_b<TB, VW>(b[j][0], reinterpret_cast<const TB *>(rbase + o[j].x));
+        if (N > 1) load_b<TB, VW>(b[j][1], reinterpret_cast<const TB *>(rbase + o[j].y));
+        if (N > 2) load_b<TB, VW>(b[j][2], reinterpret_cast<const TB *>(rbase + o[j].z));
+        if (N > 3) load_b<TB, VW>(b[j][3], reinterpret_cast<const TB *>(rbase + o[j].w));
+    }
+#pragma unroll
+    for (int j = 0; j < R; j++)
+#pragma unroll
+        for (int k = 0; k < VW; k++) {
+            float c = out.acc[j][k];
+            c = c + v[j].x * b[j][0][k];
+            if (N > 1) c = c + v[j].y * b[j][1][k];
+            if (N > 2) c = c + v[j].z * b[j][2][k];
+            if (N > 3) c = c + v[j].w * b[j][3][k];
+            out.acc[j][k] = c;
+        }
+}
+
 template <typename TB, int VW, int L, int R, int NT>
 __device__ __forceinline__ void compute_tile(const RingLaunch &a, const unsigned char *ring,
-                                             const MetaBuf &s, int nrows, TileRows<VW, R> &out) {
+                                             const MetaBuf &s, int nrows, int lmax,
+                                             TileRows<VW, R> &out) {
     constexpr int G = NT / L;
     const int t = threadIdx.x, g = t / L;
     const int col0 = (t % L) * VW;
@@ -265,33 +304,15 @@ __device__ __forceinline__ void compute_tile(const RingLaunch &a, const unsigned
 #pragma unroll
         for (int k = 0; k < VW; k++) out.acc[j][k] = 0.f;
     }
-    for (int p = 0; p < len; p += 4) {
-        int4 o[R];
-        float4 v[R];
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            o[j] = *reinterpret_cast<const int4 *>(s.offs + base[j] + p);
-            v[j] = *reinterpret_cast<const float4 *>(s.vals + base[j] + p);
-        }
-        float b[R][4][VW];
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            load_b<TB, VW>(b[j][0], reinterpret_cast<const TB *>(rbase + o[j].x));
-            load_b<TB, VW>(b[j][1], reinterpret_cast<const TB *>(rbase + o[j].y));
-            load_b<TB, VW>(b[j][2], reinterpret_cast<const TB *>(rbase + o[j].z));
-            load_b<TB, VW>(b[j][3], reinterpret_cast<const TB *>(rbase + o[j].w));
-        }
-#pragma unroll
-        for (int j = 0; j < R; j++)
-#pragma unroll
-            for (int k = 0; k < VW; k++) {
-                float c = out.acc[j][k];
-                c = c + v[j].x * b[j][0][k];
-                c = c + v[j].y * b[j][1][k];
-                c = c + v[j].z * b[j][2][k];
-                c = c + v[j].w * b[j][3][k];
-                out.acc[j][k] = c;
-            }
+    // whole steps of four, then the entries of the longest row past them
+    // (lmax is wave-uniform: the branches below are scalar)
+    const int full = lmax & ~3;
+    for (int p = 0; p < full; p += 4) ring_step<TB, VW, R, 4>(rbase, s, base, p, out);
+    switch (lmax - full) {
+    case 1: ring_step<TB, VW, R, 1>(rbase, s, base, full, out); break;
+    case 2: ring_step<TB, VW, R, 2>(rbase, s, base, full, out); break;
+    case 3: ring_step<TB, VW, R, 3>(rbase, s, base, full, out); break;
+    default: break;
     }
 }
 
@@ -351,12 +372,13 @@ void k_ring(RingLaunch a) {
     MetaRegs m;
     FillEnt<FR> ent, cur;
     FillRegs<FR> f;
-    int nrows;
+    int nrows, lmax;
     {   // tile 0, synchronously; tile 1's loads in flight; tile 2's entries
         meta_load(a, tiles[0], m);
         fill_sync<TB, FR>(a, rsm, pm, f0, fo(1), 0, row_bytes);
         meta_store(meta_buf(rsm, lay, 0), m, row_bytes);
         nrows = m.nrows;
+        lmax = m.lmax;
         // (for a one- or two-tile stream: its last tile's, never stored)
         const int k1 = min(1, ns - 1), k2 = min(2, ns - 1);
         fill_entries<FR>(a, pm, f0 + fo(k1), fo(k1 + 1) - fo(k1), 0, ent);
@@ -367,13 +389,14 @@ void k_ring(RingLaunch a) {
     }
     TileRows<VW, R> out;                  // the previous tile's rows
     for (int k = 0; k < ns; k++) {
-        int nrows_next = 0;
+        int nrows_next = 0, lmax_next = 0;
         if (k + 1 < ns) {
             // (A) tile k + 1 into LDS: its slots are none that tile k reads,
             // and tile k - 1, whose metadata buffer it takes, is done
             fill_store<FR>(rsm, f);
             meta_store(meta_buf(rsm, lay, (k + 1) & 1), m, row_bytes);
             nrows_next = m.nrows;
+            lmax_next = m.lmax;
         }
         // (B) tile k + 2's loads, tile k + 3's fill-list entries.  Issued on
         // every iteration, for the stream's last tile past its end (loaded,
@@ -390,8 +413,10 @@ void k_ring(RingLaunch a) {
         meta_load(a, tiles[k2], m);
         fill_entries<FR>(a, pm, f0 + fo(k3), fo(k3 + 1) - fo(k3), 0, ent);
         // (C) tile k
-        compute_tile<TB, VW, L, R, NT>(a, rsm, meta_buf(rsm, lay, k & 1), nrows, out);
+        compute_tile<TB, VW, L, R, NT>(a, rsm, meta_buf(rsm, lay, k & 1), nrows,
+                                       __builtin_amdgcn_readfirstlane(lmax), out);
         nrows = nrows_next;
+        lmax = lmax_next;
         __syncthreads();                  // (E)
     }
     store_rows<VW, R>(a, out);            // the last tile's rows

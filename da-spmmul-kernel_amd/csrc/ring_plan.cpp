@@ -242,11 +242,12 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
     for (int t = 0; t < ntiles; t++) {
         const int q0 = o.tile_off[(size_t)t], q1 = o.tile_off[(size_t)t + 1];
         const int p0 = (int)pos;
-        int rlen = 0;                                   // the tile's padded row length
+        int rlen = 0;                                   // the tile's longest row, then padded
         for (int q = q0; q < q1; q++) {
             const int r = o.rows[(size_t)q];
             rlen = std::max(rlen, rp[(size_t)r + 1] - rp[(size_t)r]);
         }
+        const int lmax = rlen;
         rlen = (rlen + 3) / 4 * 4;
         p.pad_entries += (int64_t)rlen * (q1 - q0);
         for (int q = q0; q < q1; q++) {
@@ -277,7 +278,7 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
         const bool first = std::binary_search(o.stream_off.begin(), o.stream_off.end(), t);
         if (!first) p.fill_max = std::max(p.fill_max, nf);
         p.nnz_max = std::max(p.nnz_max, (int)pos - p0);
-        p.tiles[(size_t)t] = make_int4(p0, (int)pos, q0, q1);
+        p.tiles[(size_t)t] = make_int4(p0, lmax, q0, q1);
     }
     p.total_fills = (int64_t)p.fills.size();
     for (int s = 0; s < nstreams; s++) {

@@ -7,6 +7,7 @@
 // appears once, and that the limits the kernel relies on hold (slots <
 // capacity, tiles' nonzero ranges 8-aligned).  Then it evaluates C through the plan in the
 // kernel's order and compares it bit for bit with the row-by-row host loop.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -118,9 +119,23 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
                 fills++;
             }
             const int4 td = p.tiles[(size_t)t];
-            if (td.x % 8 || td.y % 8 || td.y - td.x > p.nnz_max || td.w - td.z > p.rows_max) {
+            // td.y: the longest row; the tile's entries: rows x (td.y rounded
+            // to 4), rounded to 8
+            const int ntr = td.w - td.z, plen = (td.y + 3) / 4 * 4;
+            const int span = (ntr * plen + 7) / 8 * 8;
+            if (td.x % 8 || span > p.nnz_max || ntr > p.rows_max ||
+                (t + 1 < (int)p.tiles.size() && p.tiles[(size_t)t + 1].x < td.x + span)) {
                 std::printf("%s: tile %d range\n", name, t);
                 return 3;
+            }
+            int longest = 0;
+            for (int q = td.z; q < td.w; q++) {
+                const int r = p.row_info[(size_t)q].x;
+                longest = std::max(longest, a.rp[(size_t)r + 1] - a.rp[(size_t)r]);
+            }
+            if (longest != td.y) {
+                std::printf("%s: tile %d longest row %d, descriptor says %d\n", name, t, longest, td.y);
+                return 12;
             }
             int prev = 0;
             for (int q = td.z; q < td.w; q++) {
@@ -132,7 +147,7 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
                 const int padded = ri.y - prev;
                 // every row of the tile padded to the same multiple of 4
                 const int first = p.row_info[(size_t)td.z].y;
-                if (padded % 4 || padded < len || padded != first || td.x + ri.y > td.y) {
+                if (padded % 4 || padded < len || padded != first || ri.y > span) {
                     std::printf("%s: row %d end\n", name, r);
                     return 5;
                 }
