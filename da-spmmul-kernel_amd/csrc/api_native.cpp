@@ -380,12 +380,7 @@ int shared_row_stride2(dspmm_matrix_s *A) {
     return std::max(1, A->rb_stride2);
 }
 
-dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPlan **out,
-                             bool build) {
-    const auto key = std::make_tuple(w, h, zchunk);
-    auto it = A->ring_plans.find(key);
-    if (it != A->ring_plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
-    if (!build) return DSPMM_STATUS_NOT_PREPARED;
+RingShape ring_shape(dspmm_matrix_s *A, int w, int h, int zchunk) {
     RingShape sh;
     sh.w = w;
     sh.h = h;
@@ -400,6 +395,16 @@ dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPl
         if (sh.w > sh.s_big) sh.s_big = 1;
     }
     sh.slots = 0;                   // the largest tile's distinct columns, rounded up
+    return sh;
+}
+
+dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPlan **out,
+                             bool build) {
+    const auto key = std::make_tuple(w, h, zchunk);
+    auto it = A->ring_plans.find(key);
+    if (it != A->ring_plans.end()) { *out = &it->second; return DSPMM_STATUS_SUCCESS; }
+    if (!build) return DSPMM_STATUS_NOT_PREPARED;
+    const RingShape sh = ring_shape(A, w, h, zchunk);
     RingPlanHost hp;
     RingPlan p;
     p.usable = build_ring_plan(A->h_rowPtr, A->h_colIdx, A->h_values, A->nr, A->nc, sh, hp);
@@ -440,6 +445,39 @@ dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPl
     auto res = A->ring_plans.emplace(key, p);
     *out = &res.first->second;
     return DSPMM_STATUS_SUCCESS;
+}
+
+void ring_bind_plan(dspmm_matrix_s *A, const RingPlan &plan, RingLaunch &a) {
+    a.tiles = plan.d_tiles;
+    a.stream_off = plan.d_streams;
+    a.row_info = plan.d_rows;
+    a.fill_off = plan.d_fill_off;
+    a.fills = plan.d_fills;
+    a.pslot = plan.d_pslot;
+    a.pval = plan.d_pval;
+    a.nstreams = plan.nstreams;
+    a.slots = plan.slots;
+    a.rows_max = plan.rows_max;
+    a.nnz_max = plan.nnz_max;
+    a.stream_tiles_max = plan.stream_tiles_max;
+    a.stream_fills_max = plan.stream_fills_max;
+    a.fill_max = plan.fill_max;
+    a.nr = A->nr;
+    a.nc = A->nc;
+    a.plan_entries = plan.entries;
+    a.plan_fills = plan.fill_entries;
+}
+
+int ring_auto_z(dspmm_matrix_s *A, int w, int h) {
+    const auto key = std::make_pair(w, h);
+    auto it = A->ring_auto.find(key);
+    if (it != A->ring_auto.end()) return it->second;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int z = ring_auto_chunk(A->nr, ring_shape(A, w, h, 1), cus);
+    A->ring_auto.emplace(key, z);
+    return z;
 }
 
 dspmm_status_t get_plan(dspmm_matrix_s *A, int S, NbPlan **out, int kind, int stride,
@@ -834,7 +872,6 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         if (layout != DENSE_ROW_MAJOR || A->nnz == 0) return DSPMM_STATUS_NOT_SUPPORTED;
         const int w = cfg.tile_nnz > 0 ? cfg.tile_nnz : 8;
         const int h = cfg.unroll > 0 ? cfg.unroll : 4;
-        const int z = cfg.row_stride > 0 ? cfg.row_stride : 25;
         if (w > 256 || h > 256 || w * h > 256) return DSPMM_STATUS_NOT_SUPPORTED;
         dspmm_config c = cfg;
         c.tile_nnz = 0;
@@ -843,37 +880,21 @@ dspmm_status_t run_ex(dspmm_matrix_s *A, DenseLayout layout, int nv, const void 
         const Slab sl = make_slab(nv, c.vec);
         if (sl.count != 1) return DSPMM_STATUS_NOT_SUPPORTED;          // one slab only
         if (A->rb_stride < 1 && !build) return DSPMM_STATUS_NOT_PREPARED;
-        RingPlan *plan = nullptr;
-        dspmm_status_t s = get_ring_plan(A, w, h, z, &plan, build);
-        if (s != DSPMM_STATUS_SUCCESS) return s;
-        if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
         RingLaunch a;
-        a.tiles = plan->d_tiles;
-        a.stream_off = plan->d_streams;
-        a.row_info = plan->d_rows;
-        a.fill_off = plan->d_fill_off;
-        a.fills = plan->d_fills;
-        a.pslot = plan->d_pslot;
-        a.pval = plan->d_pval;
         a.B = B;
         a.C = C;
         a.ldb = ldb;
         a.ldc = ldc;
         a.nv = nv;
-        a.nstreams = plan->nstreams;
-        a.slots = plan->slots;
-        a.rows_max = plan->rows_max;
-        a.nnz_max = plan->nnz_max;
-        a.stream_tiles_max = plan->stream_tiles_max;
-        a.stream_fills_max = plan->stream_fills_max;
-        a.fill_max = plan->fill_max;
-        a.nr = A->nr;
-        a.nc = A->nc;
-        a.plan_entries = plan->entries;
-        a.plan_fills = plan->fill_entries;
         a.vw = c.vec;
         a.lanes = sl.lanes;
         a.bf16 = bf16;
+        const int z = cfg.row_stride > 0 ? cfg.row_stride : ring_auto_z(A, w, h);
+        RingPlan *plan = nullptr;
+        dspmm_status_t s = get_ring_plan(A, w, h, z, &plan, build);
+        if (s != DSPMM_STATUS_SUCCESS) return s;
+        if (!plan->usable) return DSPMM_STATUS_NOT_SUPPORTED;
+        ring_bind_plan(A, *plan, a);
         if (!ring_supported(a)) return DSPMM_STATUS_NOT_SUPPORTED;
         if (!launch) return DSPMM_STATUS_SUCCESS;
         s = launch_ring(a, st);
@@ -1433,9 +1454,11 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                 }
         if (stride > 1 || reuse >= kRowBlockMinReuse) {
             // B-row ring in LDS: tile shapes (w x h) and tiles per stream
+            // (0: the length that fills this GPU's workgroup slots in whole
+            // rounds)
             for (const auto &wh : {std::make_pair(8, 4), std::make_pair(8, 8),
                                    std::make_pair(16, 4), std::make_pair(4, 8)})
-                for (int z : {10, 25}) {
+                for (int z : {0, 10}) {
                     dspmm_config c{};
                     c.variant = DSPMM_VARIANT_RING;
                     c.tile_nnz = wh.first;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "ring.hpp"
 
 namespace dspmm {
 
@@ -100,6 +101,8 @@ struct dspmm_matrix_s {
     int rb_stride2 = -1;     // the best distance below it that divides it (1: none)
     // ring plans by (w, h, tiles per stream)
     std::map<std::tuple<int, int, int>, dspmm::RingPlan> ring_plans;
+    // tiles per stream chosen for tile shape (w, h) when the config leaves it to the library
+    std::map<std::pair<int, int>, int> ring_auto;
     std::map<std::tuple<int, int, int>, dspmm_config> tuned;       // (layout, nv, btype)
     std::map<std::tuple<int, int, int>, dspmm_config> prepared;    // AUTO's pick, by dspmm_matrix_prepare
     float *carry = nullptr;
@@ -139,6 +142,15 @@ int shared_row_stride2(dspmm_matrix_s *A);
 // on first use (build == false: DSPMM_STATUS_NOT_PREPARED if missing).
 dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPlan **out,
                              bool build = true);
+// The plan's shape for tile w x h (one row distance: w * h consecutive rows
+// per tile, walked along that distance).  Reads the sampled row distances.
+RingShape ring_shape(dspmm_matrix_s *A, int w, int h, int zchunk);
+// A launch's plan arrays and sizes from `plan`.
+void ring_bind_plan(dspmm_matrix_s *A, const RingPlan &plan, RingLaunch &a);
+// Tiles per stream of a ring launch on tile shape (w, h) when the config
+// leaves it to the library: ring_auto_chunk for the current GPU's CU count,
+// cached per shape.
+int ring_auto_z(dspmm_matrix_s *A, int w, int h);
 int cached_smax(int umax);
 int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,

@@ -47,6 +47,19 @@ struct RingPlanHost {
     int64_t pad_entries = 0;          // entries of all tiles' rows, padding included
 };
 
+// Tiles per stream (zchunk) for a GPU of `cus` CUs.  The walks of a plan
+// spread over the CUs, and the walks a CU holds share its throughput, so the
+// kernel takes about ceil(walks / cus) x (kRingFirstTileCost + tiles per
+// walk) tile times: a walk's first tile costs about kRingFirstTileCost
+// tiles more, as its B rows (a stencil's three halo planes) all load before
+// it runs, with nothing to overlap them.  Returns the length, at most
+// kRingMaxChunk, with the least such cost, the longest on ties.  E.g. 325
+// walks of 100 planes on 256 CUs: 34 (975 walks, 4 per CU, 4 x 36 = 144
+// tile times) -- 10 takes 13 x 12 = 156, 100 takes 2 x 102.
+constexpr int kRingFirstTileCost = 2;
+constexpr int kRingMaxChunk = 256;
+int ring_auto_chunk(int nr, const RingShape &sh, int cus);
+
 // False when a tile needs more distinct columns than the ring holds (or the
 // shape is invalid).
 bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,

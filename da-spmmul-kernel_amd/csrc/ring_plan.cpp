@@ -85,7 +85,39 @@ void order_consecutive(int nr, int T, int tiles_per_stream, Order &o) {
         o.stream_off.push_back(std::min(ntiles, t + tiles_per_stream));
 }
 
+// Whether a shape walks (w x h)-blocks of a grid's planes (else consecutive
+// tiles).
+bool blocked(const RingShape &sh) {
+    return sh.s_big > 1 && sh.s_mid >= 1 && sh.s_big % sh.s_mid == 0 && sh.s_mid >= sh.w &&
+           (sh.h == 1 || sh.s_mid > 1);
+}
+
 }  // namespace
+
+int ring_auto_chunk(int nr, const RingShape &sh, int cus) {
+    if (nr <= 0 || sh.w < 1 || sh.h < 1) return 1;
+    // walks side by side (one per block of a plane) and tiles along a walk
+    int64_t cols = 1, planes;
+    if (blocked(sh)) {
+        const int64_t na = sh.s_mid, nb = sh.s_big / sh.s_mid;
+        cols = ((na + sh.w - 1) / sh.w) * ((nb + sh.h - 1) / sh.h);
+        planes = ((int64_t)nr + sh.s_big - 1) / sh.s_big;
+    } else {
+        planes = ((int64_t)nr + (int64_t)sh.w * sh.h - 1) / ((int64_t)sh.w * sh.h);
+    }
+    const int64_t n = std::max(1, cus);
+    int best = 1;
+    int64_t best_cost = INT64_MAX;
+    for (int64_t z = std::min<int64_t>(planes, kRingMaxChunk); z >= 1; z--) {
+        const int64_t walks = cols * ((planes + z - 1) / z);
+        const int64_t cost = ((walks + n - 1) / n) * (kRingFirstTileCost + z);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = (int)z;
+        }
+    }
+    return best;
+}
 
 bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
                      const std::vector<float> &va, int nr, int nc, const RingShape &sh,
@@ -93,8 +125,7 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
     p = RingPlanHost{};
     if (nr <= 0 || sh.w < 1 || sh.h < 1 || sh.slots < 0 || sh.slots > 65535) return false;
     Order o;
-    if (sh.s_big > 1 && sh.s_mid >= 1 && sh.s_big % sh.s_mid == 0 && sh.s_mid >= sh.w &&
-        (sh.h == 1 || sh.s_mid > 1))
+    if (blocked(sh))
         order_blocks(nr, sh.w, sh.h, sh.s_mid, sh.s_big, std::max(1, sh.zchunk), o);
     else
         order_consecutive(nr, sh.w * sh.h, std::max(1, sh.zchunk), o);

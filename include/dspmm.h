@@ -285,8 +285,11 @@ typedef enum dspmm_variant {
                                     plan decides; a workgroup walks tiles of
                                     tile_nnz consecutive rows x unroll runs
                                     (the matrix's sampled row distances apart),
-                                    row_stride tiles per walk (0 = 25); rows
-                                    are summed like the host loop (row-exact) */
+                                    row_stride tiles per walk (0 = the length
+                                    whose walks fill this GPU's CUs x
+                                    workgroups per CU in the fewest whole
+                                    rounds); rows are summed like the host
+                                    loop (row-exact) */
 } dspmm_variant_t;
 
 typedef struct dspmm_config {
@@ -299,9 +302,10 @@ typedef struct dspmm_config {
                        [1, max(1, nr)]; 0 = consecutive rows; -1 = the
                        distance at which the matrix's sampled rows share the
                        most columns (a grid stencil's plane length; 1 when
-                       no distance stands out).  Other variants ignore it.
-                       The autotuner tries consecutive rows and that
-                       distance. */
+                       no distance stands out).  The autotuner tries
+                       consecutive rows and that distance.  RING: tiles per
+                       walk (0 = chosen for the GPU).  Other variants ignore
+                       it. */
 } dspmm_config;
 
 /* C = A * B, overwriting C (nv columns; ldb/ldc in elements, >= nv for

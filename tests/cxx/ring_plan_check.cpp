@@ -253,5 +253,37 @@ int main() {
             rc |= 1;
         }
     }
+    {
+        // tiles per stream for a GPU's CU count, worked by hand: a 100^3
+        // grid in 4 x 8 blocks is 25 x 13 = 325 walks of 100 planes
+        RingShape sh;
+        sh.w = 4;
+        sh.h = 8;
+        sh.s_mid = 100;
+        sh.s_big = 10000;
+        struct { int nr, cus, want; const char *why; } cases[] = {
+            // 34: 975 walks, ceil(975 / 256) = 4 per CU, 4 x (2 + 34) = 144;
+            // 17: 8 x 19 = 152; 15: 9 x 17 = 153; 10: 13 x 12 = 156; 50: 3 x 52
+            {1000000, 256, 34, "325 walks, 256 CUs"},
+            // 34: 2 x 36 = 72; 17: 4 x 19 = 76; 25: 3 x 27 = 81; 100: 1 x 102
+            {1000000, 512, 34, "325 walks, 512 CUs"},
+            // every split fits one walk per CU: the shortest walks
+            {1000000, 100000, 1, "one walk per CU at any length"},
+        };
+        for (const auto &c : cases) {
+            const int got = dspmm::ring_auto_chunk(c.nr, sh, c.cus);
+            std::printf("auto chunk (%s): %d (want %d)\n", c.why, got, c.want);
+            if (got != c.want) rc |= 1;
+        }
+        // consecutive tiles: 1000 rows in 32-row tiles = 32 tiles, 4 CUs:
+        // 8 per walk is 4 walks, one per CU, cost 10; 7 makes 5 walks (2 on
+        // a CU, 18); 16 costs 18
+        RingShape c1;
+        c1.w = 8;
+        c1.h = 4;
+        const int got = dspmm::ring_auto_chunk(1000, c1, 4);
+        std::printf("auto chunk (32 consecutive tiles, 4 CUs): %d (want 8)\n", got);
+        if (got != 8) rc |= 1;
+    }
     return rc;
 }

@@ -94,6 +94,80 @@ def test_ring_bit_exact(name, w, h, z):
         assert_bitwise(out, h32r, f"{name} ring w={w} h={h} bf16 nv={nv}")
 
 
+def banded_chunks(T, longest, seed=11):
+    """Consecutive T-row tiles whose longest rows are the given lengths (the
+    other rows of a tile shorter, some empty), columns inside a 64-wide band
+    near the row."""
+    rng = np.random.default_rng(seed)
+    band = 64
+    lens = []
+    for lm in longest:
+        tile = rng.integers(0, lm + 1, T)
+        tile[rng.integers(0, T)] = lm
+        lens += list(tile)
+    lens = np.asarray(lens)
+    nc = len(lens) + 2 * band
+    cols = [np.sort(rng.choice(np.arange(r + band // 2, r + 3 * band // 2), int(k), replace=False))
+            for r, k in enumerate(lens)]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate(cols).astype(np.int32)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    return Case(f"banded_chunks_{T}", len(lens), nc, rp, ci, v, 128, seed=seed)
+
+
+# (w, h, longest rows of the tiles, FP32 widths): 32-row tiles take one row
+# per 16-lane group at nv = 64, 64-row tiles two (nv = 128 would need eight
+# per 32-lane group there, which the kernel refuses)
+CHUNK_CASES = [(8, 4, [1, 3, 4, 15, 16, 17, 31, 32, 33, 47, 48], (32, 64, 128)),
+               (8, 8, [1, 4, 15, 16, 17, 31, 32], (32, 64))]
+
+
+@pytest.mark.parametrize("w,h,longest,widths", CHUNK_CASES)
+def test_ring_metadata_chunks(w, h, longest, widths):
+    """Tiles whose longest rows end anywhere in the ring kernel's four-entry
+    steps and in 16-entry runs of metadata, one and two rows per lane group,
+    lane groups of 8, 16 and 32 lanes (FP32 nv = 32, 64, 128; BF16 nv = 128):
+    bit for bit against the host loop."""
+    c = banded_chunks(w * h, longest)
+    m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=w, unroll=h, row_stride=3)
+    for nv in widths:
+        out = run(c, m, nv, cfg)
+        h32, _, _ = c.ref(RM, nv)
+        assert_bitwise(out, h32, f"{c.name} ring {w}x{h} nv={nv}")
+    d16 = D.to_bf16_device(c.d_B)
+    out = run(c, m, 128, cfg, btype=D.DSPMM_R_16BF, B=d16)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, 128, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(c.dense(128))).reshape(-1)
+    assert_bitwise(out, h32r, f"{c.name} ring {w}x{h} bf16 nv=128")
+
+
+@pytest.mark.parametrize("name", ["stencil_20", "stencil2d", "c1"])
+def test_ring_auto_stream_length(name):
+    """row_stride 0: the library picks the tiles per walk for this GPU's
+    CU count; the sums do not depend on it (bit for bit the host
+    loop, and the same bits as walks of 10 tiles)."""
+    c, m = matrix(name)
+    auto = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8, row_stride=0)
+    fixed = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8, row_stride=10)
+    for nv in (16, 64):
+        try:
+            out = run(c, m, nv, auto).copy()
+        except D.DspmmError as e:
+            assert e.status == D.STATUS_NOT_SUPPORTED and name == "c1"
+            continue
+        h32, _, _ = c.ref(RM, nv)
+        assert_bitwise(out, h32, f"{name} ring auto nv={nv}")
+        try:
+            ref = run(c, m, nv, fixed)
+        except D.DspmmError as e:
+            # walks of 10 random-row tiles may need more fills per tile
+            # than the kernel's register passes carry
+            assert e.status == D.STATUS_NOT_SUPPORTED and name == "c1"
+            continue
+        assert np.array_equal(bits(out), bits(ref)), f"{name} nv={nv}"
+
+
 def test_ring_integer_exact_and_column_major():
     c, m = matrix("int_valued")          # its 4000-long row does not fit a tile's metadata
     cfg = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=2, row_stride=8)
