@@ -419,18 +419,29 @@ dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPl
         p.fill_max = hp.fill_max;
         p.fills = hp.total_fills;
         p.entries = (int64_t)hp.pslot.size();
+        // the device copy keeps row ids only (a tile's rows all have its
+        // padded length), and one byte per slot where every slot fits
+        std::vector<int> row_ids(hp.row_info.size());
+        for (size_t i = 0; i < row_ids.size(); i++) row_ids[i] = hp.row_info[i].x;
+        p.slot_bytes = hp.zero_slot < 256 ? 1 : 2;
+        std::vector<uint8_t> slot8;
+        if (p.slot_bytes == 1) slot8.assign(hp.pslot.begin(), hp.pslot.end());
         p.fill_entries = (int64_t)hp.fills.size();
         dspmm_status_t s = upload(&p.d_tiles, hp.tiles.data(), hp.tiles.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
             s = upload(&p.d_streams, hp.stream_off.data(), hp.stream_off.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
-            s = upload(&p.d_rows, hp.row_info.data(), hp.row_info.size(), hipMemcpyHostToDevice, nullptr);
+            s = upload(&p.d_rows, row_ids.data(), row_ids.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
             s = upload(&p.d_fill_off, hp.fill_off.data(), hp.fill_off.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
             s = upload(&p.d_fills, hp.fills.data(), hp.fills.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
-            s = upload(&p.d_pslot, hp.pslot.data(), hp.pslot.size(), hipMemcpyHostToDevice, nullptr);
+            s = p.slot_bytes == 1
+                    ? upload(reinterpret_cast<uint8_t **>(&p.d_pslot), slot8.data(), slot8.size(),
+                             hipMemcpyHostToDevice, nullptr)
+                    : upload(reinterpret_cast<uint16_t **>(&p.d_pslot), hp.pslot.data(),
+                             hp.pslot.size(), hipMemcpyHostToDevice, nullptr);
         if (s == DSPMM_STATUS_SUCCESS)
             s = upload(&p.d_pval, hp.pval.data(), hp.pval.size(), hipMemcpyHostToDevice, nullptr);
         if (s != DSPMM_STATUS_SUCCESS) {
@@ -438,8 +449,8 @@ dspmm_status_t get_ring_plan(dspmm_matrix_s *A, int w, int h, int zchunk, RingPl
             return s;
         }
         p.bytes = sizeof(int4) * hp.tiles.size() + sizeof(int) * hp.stream_off.size() +
-                  sizeof(int2) * hp.row_info.size() + sizeof(int) * hp.fill_off.size() +
-                  sizeof(int2) * hp.fills.size() + sizeof(uint16_t) * hp.pslot.size() +
+                  sizeof(int) * row_ids.size() + sizeof(int) * hp.fill_off.size() +
+                  sizeof(int2) * hp.fills.size() + (size_t)p.slot_bytes * hp.pslot.size() +
                   sizeof(float) * hp.pval.size();
     }
     auto res = A->ring_plans.emplace(key, p);
@@ -454,6 +465,7 @@ void ring_bind_plan(dspmm_matrix_s *A, const RingPlan &plan, RingLaunch &a) {
     a.fill_off = plan.d_fill_off;
     a.fills = plan.d_fills;
     a.pslot = plan.d_pslot;
+    a.slot_bytes = plan.slot_bytes;
     a.pval = plan.d_pval;
     a.nstreams = plan.nstreams;
     a.slots = plan.slots;

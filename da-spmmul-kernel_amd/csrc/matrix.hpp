@@ -36,10 +36,11 @@ struct NbPlan {
 struct RingPlan {
     int4 *d_tiles = nullptr;
     int *d_streams = nullptr;
-    int2 *d_rows = nullptr;
+    int *d_rows = nullptr;           // row ids, tile after tile
     int *d_fill_off = nullptr;
     int2 *d_fills = nullptr;
-    uint16_t *d_pslot = nullptr;
+    void *d_pslot = nullptr;         // slot per entry: 1 byte when slots < 256, else 2
+    int slot_bytes = 2;
     float *d_pval = nullptr;
     int nstreams = 0, ntiles = 0, slots = 0, rows_max = 0, nnz_max = 0;
     int stream_tiles_max = 0, stream_fills_max = 0, fill_max = 0;

@@ -70,10 +70,11 @@ bool build_ring_plan(const std::vector<int> &rp, const std::vector<int> &ci,
 struct RingLaunch {
     const int4 *tiles = nullptr;
     const int *stream_off = nullptr;
-    const int2 *row_info = nullptr;
+    const int *row_info = nullptr;    // row ids of the plan's tiles, tile after tile
     const int *fill_off = nullptr;
     const int2 *fills = nullptr;
-    const uint16_t *pslot = nullptr;
+    const void *pslot = nullptr;      // slot per entry, slot_bytes wide
+    int slot_bytes = 2;               // 1 when every slot (the zero slot too) is under 256
     const float *pval = nullptr;
     const void *B = nullptr;
     float *C = nullptr;

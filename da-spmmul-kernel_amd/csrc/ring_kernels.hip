@@ -37,7 +37,7 @@ __host__ __device__ inline size_t ring_bytes16(size_t b) { return (b + 15) / 16 
 
 // LDS layout: ring (slots + the zero slot) | two metadata buffers, each:
 // ring byte offsets (int) and values (float) of a tile's nonzeros, 8 spare
-// entries each, and the tile's rows (rows_max int2) | the stream's tile
+// entries each, and the tile's row ids (rows_max int) | the stream's tile
 // descriptors (int4) and fill offsets (int) | one tile's fill-list entries
 // (fill_max + 1 int2, the last a harmless {column, slot} for lanes past
 // the end).
@@ -53,7 +53,7 @@ __host__ __device__ inline RingLayout ring_layout(int slots, int nv, bool bf16, 
     l.offs = 0;
     l.vals = 4 * ((size_t)nnz_max + 8);
     l.rows = l.vals + 4 * ((size_t)nnz_max + 8);
-    const size_t mb = ring_bytes16(l.rows + 8 * (size_t)rows_max);
+    const size_t mb = ring_bytes16(l.rows + 4 * (size_t)rows_max);
     l.meta = ring;
     l.mstride = mb;
     l.tiles = ring + 2 * mb;
@@ -82,12 +82,12 @@ __device__ __forceinline__ const T &at32(const void *base, uint32_t off) {
 struct MetaBuf {
     int *offs;               // per nonzero of the tile: ring byte offset of its B row
     float *vals;             // per nonzero: its value
-    int2 *rows;              // {row, end offset} per row of the tile
+    int *rows;               // row id per row of the tile
 };
 __device__ __forceinline__ MetaBuf meta_buf(unsigned char *lds, const RingLayout &l, int b) {
     unsigned char *base = lds + l.meta + (size_t)b * l.mstride;
     return MetaBuf{reinterpret_cast<int *>(base + l.offs), reinterpret_cast<float *>(base + l.vals),
-                   reinterpret_cast<int2 *>(base + l.rows)};
+                   reinterpret_cast<int *>(base + l.rows)};
 }
 
 // A B row is `parts` 16-byte pieces (a power of two dividing the
@@ -182,14 +182,15 @@ __device__ __forceinline__ void fill_sync(const RingLaunch &a, unsigned char *ri
 }
 
 // Metadata of one tile: thread t holds nonzeros [8t, 8t + 8) of the tile
-// (values and slots, 16-byte aligned in the plan) and row t.  Waves wholly
+// (values and slots -- one byte each where the plan's slots fit a byte,
+// else two -- aligned in the plan) and row t.  Waves wholly
 // past the tile's nonzeros (rows) load nothing; within a wave a thread past
 // the end reloads the last chunk (row).
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 struct MetaRegs {
     f32x4_t v0, v1;
-    piece_t sl;
-    int2 row;
+    piece_t sl;              // 8 slots: bytes of sl.x, sl.y, or 16-bit halves of all four
+    int row;
     int nch, nrows, lmax;
 };
 
@@ -211,21 +212,33 @@ __device__ __forceinline__ void meta_load(const RingLaunch &a, int4 td, MetaRegs
         const uint32_t v = (uint32_t)(td.x + 8 * c) * 4u, sl = (uint32_t)(td.x + 8 * c) * 2u;
         m.v0 = at32<f32x4_t>(a.pval, v);
         m.v1 = at32<f32x4_t>(a.pval, v + 16);
-        m.sl = at32<piece_t>(a.pslot, sl);
+        // two 8-byte loads either way (one-byte slots: the same eight bytes
+        // twice), so that no branch surrounds a load
+        const uint32_t s0 = a.slot_bytes == 1 ? sl >> 1 : sl, s1 = a.slot_bytes == 1 ? s0 : s0 + 8;
+        const uint2 lo = at32<uint2>(a.pslot, s0), hi = at32<uint2>(a.pslot, s1);
+        m.sl = piece_t{lo.x, lo.y, hi.x, hi.y};
     }
-    if (w0 < m.nrows) m.row = at32<int2>(a.row_info, (uint32_t)(td.z + min(t, m.nrows - 1)) * 8u);
+    if (w0 < m.nrows) m.row = at32<int>(a.row_info, (uint32_t)(td.z + min(t, m.nrows - 1)) * 4u);
 }
 
-__device__ __forceinline__ void meta_store(const MetaBuf &s, const MetaRegs &m, int row_bytes) {
+__device__ __forceinline__ void meta_store(const MetaBuf &s, const MetaRegs &m, int row_bytes,
+                                           int slot_bytes) {
     const int t = threadIdx.x;
     if (t < m.nch) {
         // slot (< 2^16) x row bytes (< 2^16): a 24-bit multiply is exact
         const unsigned rb = (unsigned)row_bytes;
         int4 *o = reinterpret_cast<int4 *>(s.offs + 8 * t);
-        o[0] = make_int4(__umul24(m.sl[0] & 0xffffu, rb), __umul24(m.sl[0] >> 16, rb),
-                         __umul24(m.sl[1] & 0xffffu, rb), __umul24(m.sl[1] >> 16, rb));
-        o[1] = make_int4(__umul24(m.sl[2] & 0xffffu, rb), __umul24(m.sl[2] >> 16, rb),
-                         __umul24(m.sl[3] & 0xffffu, rb), __umul24(m.sl[3] >> 16, rb));
+        if (slot_bytes == 1) {
+            o[0] = make_int4(__umul24(m.sl[0] & 0xffu, rb), __umul24((m.sl[0] >> 8) & 0xffu, rb),
+                             __umul24((m.sl[0] >> 16) & 0xffu, rb), __umul24(m.sl[0] >> 24, rb));
+            o[1] = make_int4(__umul24(m.sl[1] & 0xffu, rb), __umul24((m.sl[1] >> 8) & 0xffu, rb),
+                             __umul24((m.sl[1] >> 16) & 0xffu, rb), __umul24(m.sl[1] >> 24, rb));
+        } else {
+            o[0] = make_int4(__umul24(m.sl[0] & 0xffffu, rb), __umul24(m.sl[0] >> 16, rb),
+                             __umul24(m.sl[1] & 0xffffu, rb), __umul24(m.sl[1] >> 16, rb));
+            o[1] = make_int4(__umul24(m.sl[2] & 0xffffu, rb), __umul24(m.sl[2] >> 16, rb),
+                             __umul24(m.sl[3] & 0xffffu, rb), __umul24(m.sl[3] >> 16, rb));
+        }
         f32x4_t *v = reinterpret_cast<f32x4_t *>(s.vals + 8 * t);
         v[0] = m.v0;
         v[1] = m.v1;
@@ -294,13 +307,13 @@ __device__ __forceinline__ void compute_tile(const RingLaunch &a, const unsigned
     // multiple of VW)
     const int colc = min(col0, a.nv - VW);
     const unsigned char *rbase = ring + colc * (int)sizeof(TB);
-    const int len = __builtin_amdgcn_readfirstlane(s.rows[0].y);   // every row's padded length
+    const int len = (lmax + 3) & ~3;     // every row's padded length
     int base[R];
 #pragma unroll
     for (int j = 0; j < R; j++) {
         const int ie = min(g + j * G, nrows - 1);
         base[j] = (int)__umul24((unsigned)ie, (unsigned)len);   // both under 2^24
-        out.dst[j] = __umul24((unsigned)s.rows[ie].x, (unsigned)a.ldc * 4u) + colc * 4;
+        out.dst[j] = __umul24((unsigned)s.rows[ie], (unsigned)a.ldc * 4u) + colc * 4;
 #pragma unroll
         for (int k = 0; k < VW; k++) out.acc[j][k] = 0.f;
     }
@@ -376,7 +389,7 @@ void k_ring(RingLaunch a) {
     {   // tile 0, synchronously; tile 1's loads in flight; tile 2's entries
         meta_load(a, tiles[0], m);
         fill_sync<TB, FR>(a, rsm, pm, f0, fo(1), 0, row_bytes);
-        meta_store(meta_buf(rsm, lay, 0), m, row_bytes);
+        meta_store(meta_buf(rsm, lay, 0), m, row_bytes, a.slot_bytes);
         nrows = m.nrows;
         lmax = m.lmax;
         // (for a one- or two-tile stream: its last tile's, never stored)
@@ -394,7 +407,7 @@ void k_ring(RingLaunch a) {
             // (A) tile k + 1 into LDS: its slots are none that tile k reads,
             // and tile k - 1, whose metadata buffer it takes, is done
             fill_store<FR>(rsm, f);
-            meta_store(meta_buf(rsm, lay, (k + 1) & 1), m, row_bytes);
+            meta_store(meta_buf(rsm, lay, (k + 1) & 1), m, row_bytes, a.slot_bytes);
             nrows_next = m.nrows;
             lmax_next = m.lmax;
         }

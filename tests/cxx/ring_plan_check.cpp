@@ -145,9 +145,10 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
                 seen[(size_t)r] = 1;
                 const int len = a.rp[(size_t)r + 1] - a.rp[(size_t)r];
                 const int padded = ri.y - prev;
-                // every row of the tile padded to the same multiple of 4
+                // every row of the tile padded to the same length: the longest
+                // row rounded up to 4 (the kernel derives it from the descriptor)
                 const int first = p.row_info[(size_t)td.z].y;
-                if (padded % 4 || padded < len || padded != first || ri.y > span) {
+                if (padded % 4 || padded < len || padded != first || padded != plen || ri.y > span) {
                     std::printf("%s: row %d end\n", name, r);
                     return 5;
                 }
