@@ -121,11 +121,12 @@ def traffic_from_profile(tag, config):
         return None
 
 
-def prefer_profiled(D, np, m, d_B, d_C, nv, best, profiled, btype, rounds=3, reps=20):
+def prefer_profiled(D, np, m, d_B, d_C, nv, best, profiled, btype, rounds=3, reps=50):
     """Between the autotuner's pick and the configuration whose HBM traffic
     was profiled, keep the profiled one unless the pick is >2% faster
-    (medians of interleaved timing rounds): autotuning choices among
-    near-equal configurations flip with timing noise."""
+    (the better of interleaved region-timed rounds, timed the way `value`
+    is): autotuning choices among near-equal configurations flip with
+    timing noise."""
     if not profiled or profiled == best.as_dict():
         return best
     cand = [D.config_from_dict(profiled), best]
@@ -133,10 +134,11 @@ def prefer_profiled(D, np, m, d_B, d_C, nv, best, profiled, btype, rounds=3, rep
     for _ in range(rounds):
         for i, c in enumerate(cand):
             try:
-                _, per, _ = m.benchmark(d_B, d_C, nv, btype=btype, config=c, warmup=2, reps=reps)
+                mean, _, _ = m.benchmark(d_B, d_C, nv, btype=btype, config=c, warmup=5, reps=reps,
+                                         per_call=False)
             except D.DspmmError:
                 return best
-            t[i].append(float(np.median(per)))
+            t[i].append(float(mean))
     log(f"profiled config {profiled}: {min(t[0]) * 1e3:.1f} us; autotuned "
         f"{best.as_dict()}: {min(t[1]) * 1e3:.1f} us")
     return best if min(t[1]) < 0.98 * min(t[0]) else cand[0]

@@ -1465,12 +1465,21 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                     cand.push_back(c);
                 }
         if (stride > 1 || reuse >= kRowBlockMinReuse) {
-            // B-row ring in LDS: tile shapes (w x h) and tiles per stream
-            // (0: the length that fills this GPU's workgroup slots in whole
-            // rounds)
+            // B-row ring in LDS: tile shapes (w x h) and tiles per walk:
+            // the library's length for this GPU's CUs (0) and half of it
+            // (twice the walks, so the two workgroups of a CU start them
+            // out of step; measured better where one walk per workgroup
+            // slot would start them all at once)
             for (const auto &wh : {std::make_pair(8, 4), std::make_pair(8, 8),
-                                   std::make_pair(16, 4), std::make_pair(4, 8)})
-                for (int z : {0, 10}) {
+                                   std::make_pair(16, 4), std::make_pair(4, 8)}) {
+                int half;
+                {
+                    std::lock_guard<std::mutex> lk(A->mu);
+                    half = ring_auto_z(A, wh.first, wh.second) / 2;
+                }
+                const int zs[2] = {0, half};
+                for (int q = 0; q < (half >= 1 ? 2 : 1); q++) {
+                    const int z = zs[q];
                     dspmm_config c{};
                     c.variant = DSPMM_VARIANT_RING;
                     c.tile_nnz = wh.first;
@@ -1478,6 +1487,7 @@ std::vector<dspmm_config> autotune_candidates(dspmm_matrix_s *A, DenseLayout lay
                     c.row_stride = z;
                     cand.push_back(c);
                 }
+            }
         }
         if (reuse >= kCachedMinReuse) {
             dspmm_config base{};
