@@ -110,10 +110,11 @@ def collect(out_path, reps=5):
             ok = [(c, ms) for c, ms in tab if ms is not None]
             for c, ms in ok:
                 cfg = {k: getattr(c, k) for k in CFG_FIELDS}
-                # the only row stride above 1 the autotuner tries is the matrix's
-                # detected one; stored as -1 ("detected") so it transfers between
-                # matrices
-                if cfg["row_stride"] > 1:
+                # the only row stride above 1 the autotuner tries for row blocks
+                # is the matrix's detected one; stored as -1 ("detected") so it
+                # transfers between matrices (a ring's row_stride is its walk
+                # length and stays as it is)
+                if cfg["row_stride"] > 1 and cfg["variant"] == 8:
                     cfg["row_stride"] = -1
                 rows.append({"matrix": name, "nv": nv, **f, "nnz": a.nnz, **cfg, "ms": ms})
             best = min(ok, key=lambda x: x[1])

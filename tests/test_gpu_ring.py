@@ -168,6 +168,21 @@ def test_ring_auto_stream_length(name):
         assert np.array_equal(bits(out), bits(ref)), f"{name} nv={nv}"
 
 
+def test_untuned_auto_takes_the_ring_on_a_stencil():
+    """Without autotuning, AUTO on a 27-point stencil at nv = 64 resolves (by
+    the learned selector, whose corpus times the ring on stencils) to the
+    B-row ring, and the call is bit for bit the host loop."""
+    a = W.c3_matrix(48)
+    c = Case("stencil48", a.nr, a.nc, a.rowPtr, a.colIdx, a.values, 64, seed=5)
+    m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+    cfg = m.resolve(64)
+    note(f"stencil48 nv=64 AUTO resolves to {cfg.as_dict()}")
+    assert cfg.variant == D.VARIANT_RING
+    out = run(c, m, 64, None)
+    h32, _, _ = c.ref(RM, 64)
+    assert_bitwise(out, h32, "stencil48 AUTO nv=64")
+
+
 def test_ring_integer_exact_and_column_major():
     c, m = matrix("int_valued")          # its 4000-long row does not fit a tile's metadata
     cfg = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=2, row_stride=8)
