@@ -1041,8 +1041,17 @@ dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_
     it = A->prepared.find(key);
     if (it != A->prepared.end()) return it->second;
     dspmm_config c;
-    if (learned && layout == DENSE_ROW_MAJOR && bt == DSPMM_R_32F && learned_select(A, nv, &c))
-        return c;
+    if (learned && layout == DENSE_ROW_MAJOR && learned_select(A, nv, &c)) {
+        // the selector was measured with FP32 B; with BF16 B only its B-row
+        // ring picks carry over (the ring takes either, with the vector width
+        // left to the operands)
+        if (bt == DSPMM_R_32F) return c;
+        if (c.variant == DSPMM_VARIANT_RING) {
+            c.vec = 0;
+            c.lanes = 0;
+            return c;
+        }
+    }
     return heuristic(A, layout, nv, bt);
 }
 

@@ -157,8 +157,9 @@ int64_t carry_ld(int nv);
 void complete_nb(dspmm_config &c, int nv, int64_t ldb, int64_t ldc, const void *B, const void *C,
                  bool bf16);
 // The configuration a call runs: cfg unless AUTO, else the tuned one cached
-// for (layout, nv, bt), else (with `learned`, row-major FP32 only) the
-// nearest-neighbour selector's pick, else the static heuristic.
+// for (layout, nv, bt), else (with `learned`, row-major only) the
+// nearest-neighbour selector's pick -- with BF16 B only a B-row ring pick --
+// else the static heuristic.
 dspmm_config resolve(dspmm_matrix_s *A, DenseLayout layout, int nv, dspmm_dtype_t bt,
                      const dspmm_config *cfg, bool learned = true);
 // The static default (no timing, no corpus).

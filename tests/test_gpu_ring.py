@@ -171,7 +171,8 @@ def test_ring_auto_stream_length(name):
 def test_untuned_auto_takes_the_ring_on_a_stencil():
     """Without autotuning, AUTO on a 27-point stencil at nv = 64 resolves (by
     the learned selector, whose corpus times the ring on stencils) to the
-    B-row ring, and the call is bit for bit the host loop."""
+    B-row ring, with FP32 and with BF16-stored B, and the call is bit for bit
+    the host loop."""
     a = W.c3_matrix(48)
     c = Case("stencil48", a.nr, a.nc, a.rowPtr, a.colIdx, a.values, 64, seed=5)
     m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
@@ -181,6 +182,14 @@ def test_untuned_auto_takes_the_ring_on_a_stencil():
     out = run(c, m, 64, None)
     h32, _, _ = c.ref(RM, 64)
     assert_bitwise(out, h32, "stencil48 AUTO nv=64")
+    # BF16-stored B: the ring pick carries over, bit for bit the host loop on
+    # the rounded B
+    assert m.resolve(64, btype=D.DSPMM_R_16BF).variant == D.VARIANT_RING
+    d16 = D.to_bf16_device(c.d_B)
+    out = run(c, m, 64, None, btype=D.DSPMM_R_16BF, B=d16)
+    h32r = O.spmm_host32(RM, c.nr, c.nc, 64, c.rowPtr, c.colIdx, c.values,
+                         O.round_bf16(c.dense(64))).reshape(-1)
+    assert_bitwise(out, h32r, "stencil48 AUTO bf16 nv=64")
 
 
 def test_ring_integer_exact_and_column_major():
