@@ -373,8 +373,10 @@ DSPMM_API dspmm_status_t dspmm_spmm_ex_resolve(dspmm_matrix_t A, DenseLayout lay
                                                dspmm_config *resolved);
 
 /* Autotuner: times the candidate variants and tile shapes on the given
- * operands (C is overwritten), caches the fastest in the handle for
- * (layout, nv, btype) and returns it. */
+ * operands (C is overwritten; reps calls each, median), times the front
+ * runners within 5% of the best again (up to four, 4 x reps calls, in both
+ * orders), caches the fastest in the handle for (layout, nv, btype) and
+ * returns it with its median time. */
 DSPMM_API dspmm_status_t dspmm_autotune(dspmm_matrix_t A, DenseLayout layout, int nv,
                                         const void *B, int64_t ldb, dspmm_dtype_t btype,
                                         float *C, int64_t ldc, int reps,
