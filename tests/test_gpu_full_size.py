@@ -56,8 +56,10 @@ def run(m, d_B, d_C, nv, cfg, btype=D.DSPMM_R_32F):
 
 def test_stencil_100_cubed_row_exact_kernels_agree():
     """27-point stencil on a 100^3 grid (M = 1e6, 26.5 M nonzeros), N = 64:
-    five row-exact kernels give the same 64 M outputs; sampled rows equal the
-    host loop; the same with BF16-stored B (three kernels)."""
+    eight row-exact kernels -- the headline B-row ring among them, at the
+    tile shapes and walk lengths the bench runs -- give the same 64 M
+    outputs; sampled rows equal the host loop; the same with BF16-stored B
+    (five kernels)."""
     a = W.c3_matrix(100)
     nv = 64
     B = W.dense_b(3, a.nc, nv)
@@ -69,7 +71,11 @@ def test_stencil_100_cubed_row_exact_kernels_agree():
             "nnz-tiled S=64": D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=64),
             "row-blocked R=2": D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4),
             "row-blocked R=4, rows a plane apart":
-                D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4, unroll=4, row_stride=10000)}
+                D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=4, unroll=4, row_stride=10000),
+            "ring 4x8, automatic walks": D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8),
+            "ring 8x4, automatic walks": D.make_config(D.VARIANT_RING, tile_nnz=8, unroll=4),
+            "ring 8x8, walks of 17":
+                D.make_config(D.VARIANT_RING, tile_nnz=8, unroll=8, row_stride=17)}
     first = None
     for name, cfg in cfgs.items():
         note(f"c3 full {name}")
@@ -85,7 +91,9 @@ def test_stencil_100_cubed_row_exact_kernels_agree():
     outs = [run(m, d16, d_C, nv, cfg, D.DSPMM_R_16BF) for cfg in
             (D.make_config(D.VARIANT_CSR_SCALAR, unroll=4),
              D.make_config(D.VARIANT_NNZ_TILED, vec=8, lanes=8, tile_nnz=32, unroll=4),
-             D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4, row_stride=10000))]
+             D.make_config(D.VARIANT_ROW_BLOCKED, tile_nnz=2, unroll=4, row_stride=10000),
+             D.make_config(D.VARIANT_RING, tile_nnz=8, unroll=8, row_stride=17),
+             D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8))]
     for o in outs[1:]:
         assert np.array_equal(bits(outs[0]), bits(o)), "c3 BF16: row kernels differ"
     rows, rp, ci, v = sample(a, 3000, 32)
