@@ -234,6 +234,14 @@ struct dspmm_server_s {
             dspmm_spmm_ex_workspace_size(m.h, DENSE_ROW_MAJOR, nv, cfg.btype, &fr.own, &fr.own_ws) ==
                 DSPMM_STATUS_SUCCESS)
             fr.own_ok = fr.f.nsplits == 0 && row_exact(m.h, nv, fr.own);
+        // the own kernel's plan (a ring plan, say) built now, at admission,
+        // not inside a launch (placeholder operands: only their alignment
+        // class and strides matter)
+        if (fr.own_ok &&
+            dspmm_matrix_prepare(m.h, DENSE_ROW_MAJOR, nv, reinterpret_cast<const void *>(256), nv,
+                                 cfg.btype, reinterpret_cast<const float *>(256), nv, &fr.own,
+                                 0) != DSPMM_STATUS_SUCCESS)
+            fr.own_ok = false;
         return &m.frags.emplace(nv, fr).first->second;
     }
 

@@ -225,3 +225,36 @@ def test_server_bf16_operands():
         err = np.abs(alone[k].astype(np.float64) - c64.reshape(-1))
         assert np.all(err <= 1e-2 * ab.reshape(-1)), f"bf16 request {k}: beyond 1e-2"
     srv.close()
+
+
+def test_server_serves_a_stencil_on_its_ring():
+    """A 36^3 27-point stencil beside random matrices: its own AUTO kernel is
+    the B-row ring (plan built at admission), requests of it alone or mixed
+    equal the host loop bit for bit on the device-generated B, and in a
+    burst they share batches with the other matrices."""
+    pool = make_pool(n=4, seed=61)
+    pool.append((W.c3_matrix(36), 64))
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16)
+    mids = register(srv, pool)
+    si = len(pool) - 1
+    a, nv = pool[si]
+    m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
+    assert m.resolve(nv).variant == D.VARIANT_RING
+    m.close()
+    reqs, at, mi_of = [], [], {}
+    rng = np.random.default_rng(4)
+    for k in range(40):
+        mi = si if k % 4 == 0 else int(rng.integers(len(pool) - 1))
+        reqs.append(srv.request(k, mids[mi], pool[mi][1], 900 + k))
+        at.append(k * 10e-6)
+        mi_of[k] = mi
+    srv.replay(reqs, at)
+    comps = srv.poll()
+    assert len(comps) == len(reqs)
+    got = results(srv, comps, pool, mi_of)
+    for k in range(0, 40, 4):
+        B = D.seeded_uniform(900 + k, a.nc * nv)
+        h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
+        assert np.array_equal(bits(got[k]), bits(h32)), f"stencil request {k}"
+    note(f"stencil in the mix: stats {srv.stats()}")
+    srv.close()
