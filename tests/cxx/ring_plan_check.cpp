@@ -5,7 +5,7 @@
 // writes a tile's fills while the previous tile may still be computing),
 // that the plan's values and row ends follow the CSR arrays, that every row
 // appears once, and that the limits the kernel relies on hold (slots <
-// capacity, tiles' nonzero ranges 8-aligned).  Then it evaluates C through the plan in the
+// capacity, tiles' nonzero ranges 8-aligned and inside the arrays).  Then it evaluates C through the plan in the
 // kernel's order and compares it bit for bit with the row-by-row host loop.
 #include <algorithm>
 #include <climits>
@@ -32,26 +32,28 @@ static float hval(uint64_t i) {
     return (float)((z >> 40) & 0xffff) / 65536.0f - 0.5f;
 }
 
-// 27-point stencil on an n^3 grid, x fastest.
-static Csr stencil(int n) {
+// 27-point stencil on an nx x ny x nz grid, x fastest.
+static Csr stencil3(int nx, int ny, int nz) {
     Csr a;
-    a.nr = a.nc = n * n * n;
+    a.nr = a.nc = nx * ny * nz;
     a.rp.push_back(0);
-    for (int z = 0; z < n; z++)
-        for (int y = 0; y < n; y++)
-            for (int x = 0; x < n; x++) {
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
                 for (int dz = -1; dz <= 1; dz++)
                     for (int dy = -1; dy <= 1; dy++)
                         for (int dx = -1; dx <= 1; dx++) {
                             const int xx = x + dx, yy = y + dy, zz = z + dz;
-                            if (xx < 0 || yy < 0 || zz < 0 || xx >= n || yy >= n || zz >= n) continue;
-                            a.ci.push_back(xx + n * (yy + n * zz));
+                            if (xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+                            a.ci.push_back(xx + nx * (yy + ny * zz));
                             a.va.push_back(hval(a.ci.size()));
                         }
                 a.rp.push_back((int)a.ci.size());
             }
     return a;
 }
+
+static Csr stencil(int n) { return stencil3(n, n, n); }
 
 // Random rows of 0..maxlen sorted distinct columns (some empty).
 static Csr random_rows(int nr, int nc, int maxlen, uint64_t seed) {
@@ -123,7 +125,11 @@ static int check(const char *name, const Csr &a, const RingShape &sh, int nv) {
             // to 4), rounded to 8
             const int ntr = td.w - td.z, plen = (td.y + 3) / 4 * 4;
             const int span = (ntr * plen + 7) / 8 * 8;
+            // the kernel loads whole 8-entry chunks: the last one must lie
+            // inside the arrays too
             if (td.x % 8 || span > p.nnz_max || ntr > p.rows_max ||
+                (size_t)td.x + (size_t)span > p.pval.size() ||
+                (size_t)td.x + (size_t)span > p.pslot.size() ||
                 (t + 1 < (int)p.tiles.size() && p.tiles[(size_t)t + 1].x < td.x + span)) {
                 std::printf("%s: tile %d range\n", name, t);
                 return 3;
@@ -253,6 +259,93 @@ int main() {
             std::printf("a ring smaller than a tile's columns must be refused\n");
             rc |= 1;
         }
+    }
+    {
+        // grids smaller than one tile, a grid whose line and plane lengths
+        // no tile dimension divides (partial tiles on every edge), and the
+        // shapes the library picks for it
+        struct { int nx, ny, nz, w, h, z; } g[] = {
+            {2, 2, 2, 4, 8, 0}, {3, 3, 3, 4, 8, 0}, {3, 3, 3, 8, 8, 5}, {1, 1, 5, 4, 4, 3},
+            {7, 9, 11, 4, 8, 0}, {7, 9, 11, 8, 4, 4}, {13, 5, 3, 8, 8, 17}, {37, 2, 2, 16, 4, 2}};
+        for (const auto &q : g) {
+            Csr a = stencil3(q.nx, q.ny, q.nz);
+            RingShape sh;
+            sh.w = q.w;
+            sh.h = q.h;
+            sh.s_mid = q.nx;
+            sh.s_big = q.nx * q.ny;
+            sh.zchunk = q.z > 0 ? q.z : dspmm::ring_auto_chunk(a.nr, sh, 256);
+            char name[96];
+            std::snprintf(name, sizeof name, "grid %dx%dx%d w%d h%d z%d", q.nx, q.ny, q.nz, q.w, q.h,
+                          sh.zchunk);
+            rc |= check(name, a, sh, 8);
+        }
+    }
+    {
+        // no structure: one row; nine rows (a partial tile); most rows
+        // empty; many more columns than rows; a few columns shared by many
+        // rows (one B row read by every tile)
+        struct { int nr, nc, maxlen; uint64_t seed; int w, h; const char *name; } r[] = {
+            {1, 50, 20, 3, 8, 4, "one row"},
+            {9, 9, 9, 5, 4, 8, "nine rows"},
+            {4000, 100000, 30, 7, 8, 4, "wide: nc = 25 nr"},
+            {6000, 12, 12, 9, 4, 8, "narrow: 12 columns"},
+        };
+        for (const auto &q : r) {
+            Csr a = random_rows(q.nr, q.nc, q.maxlen, q.seed);
+            RingShape sh;
+            sh.w = q.w;
+            sh.h = q.h;
+            sh.zchunk = dspmm::ring_auto_chunk(a.nr, sh, 256);
+            rc |= check(q.name, a, sh, 8);
+        }
+        Csr e = random_rows(3000, 3000, 6, 13);      // keep one row in 100
+        Csr s;
+        s.nr = e.nr;
+        s.nc = e.nc;
+        s.rp.push_back(0);
+        for (int i = 0; i < e.nr; i++) {
+            if (i % 100 == 17)
+                for (int q = e.rp[(size_t)i]; q < e.rp[(size_t)i + 1]; q++) {
+                    s.ci.push_back(e.ci[(size_t)q]);
+                    s.va.push_back(e.va[(size_t)q]);
+                }
+            s.rp.push_back((int)s.ci.size());
+        }
+        RingShape sh;
+        sh.w = 8;
+        sh.h = 4;
+        sh.zchunk = 4;
+        rc |= check("mostly empty rows", s, sh, 8);
+    }
+    {
+        // the library takes its row distances from sampled rows, so an
+        // unstructured matrix may come with any pair: each (w, h, s_mid,
+        // s_big) as the library turns it into a shape (one distance when the
+        // smaller is under w) must give an exact plan, blocked or not
+        Csr a = random_rows(2500, 2600, 24, 21);
+        const int dist[][2] = {{8, 64}, {16, 48}, {5, 37}, {40, 40}, {1, 300}, {600, 120},
+                               {4, 2400}, {2, 2}};
+        for (const auto &d : dist)
+            for (int w : {4, 8})
+                for (int h : {4, 8}) {
+                    RingShape sh;
+                    sh.w = w;
+                    sh.h = h;
+                    sh.s_mid = d[0];
+                    sh.s_big = d[1];
+                    if (sh.s_big > 1 && (sh.s_mid < w || sh.s_mid == 1)) {
+                        sh.s_mid = sh.s_big;
+                        sh.w = w * h;
+                        sh.h = 1;
+                        if (sh.w > sh.s_big) sh.s_big = 1;
+                    }
+                    sh.zchunk = dspmm::ring_auto_chunk(a.nr, sh, 256);
+                    char name[96];
+                    std::snprintf(name, sizeof name, "random, distances %d %d, w%d h%d", d[0], d[1],
+                                  w, h);
+                    rc |= check(name, a, sh, 4);
+                }
     }
     {
         // tiles per stream for a GPU's CU count, worked by hand: a 100^3

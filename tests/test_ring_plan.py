@@ -4,8 +4,11 @@ simulated ring, checks that each nonzero's slot holds its column when its
 tile runs, that rows appear once and the kernel's limits hold, and that C
 evaluated through the plan equals the host loop bit for bit -- on grid
 stencils (ragged tile shapes, one and two row distances, a fixed larger
-ring), random rows in consecutive tiles, and a ring too small to be
-accepted."""
+ring, grids smaller than one tile, non-cubic grids), random rows in
+consecutive tiles (one row, a partial tile, far more columns than rows, a
+dozen columns shared by every row, mostly empty rows), random rows under
+arbitrary row-distance pairs (the library samples them, so an unstructured
+matrix may bring any), and a ring too small to be accepted."""
 import os
 import shutil
 import subprocess
@@ -29,4 +32,4 @@ def test_ring_plan_replays_exactly(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = [l for l in out.stdout.splitlines() if ": ok," in l]
-    assert len(lines) == 17, out.stdout
+    assert len(lines) == 62, out.stdout
