@@ -224,6 +224,100 @@ def test_ring_runs_unstructured_small_tiles():
         assert_bitwise(out, h32, f"c1 ring 4x4 z3 nv={nv}")
 
 
+def grid3(nx, ny, nz, seed=9):
+    """27-point stencil on an nx x ny x nz grid (x fastest)."""
+    rows, cols = [], []
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            xx, yy, zz = x + dx, y + dy, z + dz
+                            if 0 <= xx < nx and 0 <= yy < ny and 0 <= zz < nz:
+                                rows.append(x + nx * (y + ny * z))
+                                cols.append(xx + nx * (yy + ny * zz))
+    n = nx * ny * nz
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    v = np.random.default_rng(seed).uniform(-1, 1, len(cols)).astype(np.float32)
+    return Case(f"grid{nx}x{ny}x{nz}", n, n, rp, np.asarray(cols, np.int32), v, 64, seed=seed)
+
+
+def random_rows(name, nr, nc, maxlen, seed, keep_every=1):
+    """Rows of 0..maxlen sorted distinct random columns; with keep_every k,
+    all but one row in k are empty."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, maxlen + 1, nr)
+    lens = np.minimum(lens, nc)
+    if keep_every > 1:
+        lens[np.arange(nr) % keep_every != keep_every // 2] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    parts = [np.sort(rng.choice(nc, int(k), replace=False)) for k in lens if k]
+    ci = (np.concatenate(parts) if parts else np.zeros(0)).astype(np.int32)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    return Case(name, nr, nc, rp, ci, v, 64, seed=seed)
+
+
+EDGE = {
+    "grid2x2x2": lambda: grid3(2, 2, 2),          # smaller than one tile
+    "grid3x3x3": lambda: grid3(3, 3, 3),
+    "grid1x1x5": lambda: grid3(1, 1, 5),          # a line: one row distance
+    "grid7x9x11": lambda: grid3(7, 9, 11),        # partial tiles on every edge
+    "grid13x5x3": lambda: grid3(13, 5, 3),
+    "grid37x2x2": lambda: grid3(37, 2, 2),
+    "one_row": lambda: random_rows("one_row", 1, 50, 20, 3),
+    "nine_rows": lambda: random_rows("nine_rows", 9, 9, 9, 5),
+    "wide": lambda: random_rows("wide", 4000, 100000, 30, 7),     # nc = 25 nr
+    "narrow": lambda: random_rows("narrow", 6000, 12, 12, 9),     # every row shares 12 B rows
+    "mostly_empty": lambda: random_rows("mostly_empty", 3000, 3000, 6, 13, keep_every=100),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_ring_edge_shapes(name):
+    """Tiny, ragged and lopsided matrices through the ring: bit for bit the
+    host loop at every tile shape, with FP32 and BF16-stored B.  A shape
+    whose tile needs more B rows than LDS holds is refused; grids never
+    are."""
+    c = EDGE[name]()
+    m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+    d16 = D.to_bf16_device(c.d_B)
+    ran = 0
+    for w, h, z in SHAPES + [(4, 8, 0)]:
+        cfg = D.make_config(D.VARIANT_RING, tile_nnz=w, unroll=h, row_stride=z)
+        for nv in (4, 16, 64):
+            what = f"{name} ({c.nr}x{c.nc}, nnz {c.nnz}) ring w={w} h={h} z={z} nv={nv}"
+            note(what)
+            try:
+                out = run(c, m, nv, cfg)
+            except D.DspmmError as e:
+                assert e.status == D.STATUS_NOT_SUPPORTED and not name.startswith("grid"), what
+                continue
+            h32, _, _ = c.ref(RM, nv)
+            assert_bitwise(out, h32, what)
+            ran += 1
+            if nv == 64:
+                out = run(c, m, nv, cfg, btype=D.DSPMM_R_16BF, B=d16)
+                h32r = O.spmm_host32(RM, c.nr, c.nc, nv, c.rowPtr, c.colIdx, c.values,
+                                     O.round_bf16(c.dense(nv))).reshape(-1)
+                assert_bitwise(out, h32r, what + " bf16")
+    assert ran >= 5, name
+    m.close()
+
+
+def test_ring_refuses_an_empty_matrix():
+    """No nonzeros: nothing to stage, the ring refuses and AUTO writes zeros."""
+    c = Case("empty", 40, 30, np.zeros(41, np.int32), np.zeros(0, np.int32),
+             np.zeros(0, np.float32), 64)
+    m = D.Matrix(c.nr, c.nc, c.rowPtr, c.colIdx, c.values)
+    with pytest.raises(D.DspmmError) as e:
+        run(c, m, 16, D.make_config(D.VARIANT_RING))
+    assert e.value.status == D.STATUS_NOT_SUPPORTED
+    out = run(c, m, 16, None)
+    assert np.array_equal(bits(out), np.zeros(c.nr * 16, np.uint32))
+    m.close()
+
+
 def test_ring_refuses_what_it_cannot_take():
     c, m = matrix("stencil_20")
     cfg = D.make_config(D.VARIANT_RING)
