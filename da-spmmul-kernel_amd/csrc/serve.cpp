@@ -143,10 +143,10 @@ struct Slot {
     hipEvent_t ev = nullptr;
     bool busy = false;
     std::vector<Job> jobs;
-    ItemDesc *d_items = nullptr;
-    int4 *d_segs = nullptr;
-    int *d_xoff = nullptr;
-    int *d_prefix = nullptr;
+    // a batch's schedule: items, segments, XCD offsets, split prefix, each
+    // section 256-byte aligned, packed in the pinned stage as in the device
+    // block, so one copy uploads it
+    char *d_sched = nullptr;
     char *h_stage = nullptr;   // pinned
     size_t stage_bytes = 0;
 };
@@ -419,25 +419,24 @@ struct dspmm_server_s {
         char *p = sl.h_stage;
         const size_t bi = sizeof(ItemDesc) * (size_t)count, bs = sizeof(int4) * sch.segs.size(),
                      bx = sizeof(int) * sch.xoff.size(), bp = sizeof(int) * sch.split_prefix.size();
+        const size_t os = round_up(bi, 256), ox = os + round_up(bs, 256), op = ox + round_up(bx, 256);
+        if (op + bp > sl.stage_bytes) return DSPMM_STATUS_NOT_SUPPORTED;
         std::memcpy(p, sch.items.data(), bi);
-        std::memcpy(p + bi, sch.segs.data(), bs);
-        std::memcpy(p + bi + bs, sch.xoff.data(), bx);
-        std::memcpy(p + bi + bs + bx, sch.split_prefix.data(), bp);
-        dspmm_status_t st = hip_status(hipMemcpyAsync(sl.d_items, p, bi, hipMemcpyHostToDevice, sl.st));
-        if (st == DSPMM_STATUS_SUCCESS && bs)
-            st = hip_status(hipMemcpyAsync(sl.d_segs, p + bi, bs, hipMemcpyHostToDevice, sl.st));
-        if (st == DSPMM_STATUS_SUCCESS)
-            st = hip_status(hipMemcpyAsync(sl.d_xoff, p + bi + bs, bx, hipMemcpyHostToDevice, sl.st));
-        if (st == DSPMM_STATUS_SUCCESS)
-            st = hip_status(hipMemcpyAsync(sl.d_prefix, p + bi + bs + bx, bp, hipMemcpyHostToDevice, sl.st));
+        std::memcpy(p + os, sch.segs.data(), bs);
+        std::memcpy(p + ox, sch.xoff.data(), bx);
+        std::memcpy(p + op, sch.split_prefix.data(), bp);
+        // one upload per batch (a small copy costs microseconds of the
+        // stream's time whatever its size)
+        const dspmm_status_t st =
+            hip_status(hipMemcpyAsync(sl.d_sched, p, op + bp, hipMemcpyHostToDevice, sl.st));
         if (st != DSPMM_STATUS_SUCCESS) return st;
         BatchLaunch a;
-        a.items = sl.d_items;
-        a.segs = sl.d_segs;
-        a.xoff = sl.d_xoff;
+        a.items = reinterpret_cast<const ItemDesc *>(sl.d_sched);
+        a.segs = reinterpret_cast<const int4 *>(sl.d_sched + os);
+        a.xoff = reinterpret_cast<const int *>(sl.d_sched + ox);
         a.lds_ints_per_wave = sch.lds_ints;
         a.nblocks = schedule_blocks(sch, 0);
-        a.split_prefix = sl.d_prefix;
+        a.split_prefix = reinterpret_cast<const int *>(sl.d_sched + op);
         a.count = count;
         a.nsplits = sch.nsplits;
         a.bf16 = cfg.btype == DSPMM_R_16BF;
@@ -544,11 +543,8 @@ dspmm_status_t dspmm_server_create(dspmm_server_t *out, const dspmm_server_confi
         if (cfg->plain) continue;
         const size_t bi = sizeof(ItemDesc) * (size_t)items_cap, bs = sizeof(int4) * (size_t)s->seg_cap,
                      bx = sizeof(int) * (2 * kNumXcd + 1), bp = sizeof(int) * ((size_t)items_cap + 1);
-        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipMalloc((void **)&sl.d_items, bi));
-        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipMalloc((void **)&sl.d_segs, bs));
-        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipMalloc((void **)&sl.d_xoff, bx));
-        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipMalloc((void **)&sl.d_prefix, bp));
-        sl.stage_bytes = bi + bs + bx + bp;
+        sl.stage_bytes = round_up(bi, 256) + round_up(bs, 256) + round_up(bx, 256) + bp;
+        if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipMalloc((void **)&sl.d_sched, sl.stage_bytes));
         if (st == DSPMM_STATUS_SUCCESS) st = hip_status(hipHostMalloc((void **)&sl.h_stage, sl.stage_bytes));
     }
     if (st != DSPMM_STATUS_SUCCESS) {
@@ -575,10 +571,7 @@ dspmm_status_t dspmm_server_destroy(dspmm_server_t s) {
         Slot &sl = s->slots[i];
         if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.st && !(s->cfg.plain && i > 0)) (void)hipStreamDestroy(sl.st);
-        (void)hipFree(sl.d_items);
-        (void)hipFree(sl.d_segs);
-        (void)hipFree(sl.d_xoff);
-        (void)hipFree(sl.d_prefix);
+        (void)hipFree(sl.d_sched);
         if (sl.h_stage) (void)hipHostFree(sl.h_stage);
     }
     for (auto &m : s->mats)
