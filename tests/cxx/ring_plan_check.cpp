@@ -55,6 +55,26 @@ static Csr stencil3(int nx, int ny, int nz) {
 
 static Csr stencil(int n) { return stencil3(n, n, n); }
 
+// 7-point stencil on an nx x ny x nz grid (nz = 1: the 5-point stencil).
+static Csr stencil7(int nx, int ny, int nz) {
+    Csr a;
+    a.nr = a.nc = nx * ny * nz;
+    a.rp.push_back(0);
+    const int d[7][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                for (const auto &o : d) {
+                    const int xx = x + o[0], yy = y + o[1], zz = z + o[2];
+                    if (xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+                    a.ci.push_back(xx + nx * (yy + ny * zz));
+                    a.va.push_back(hval(a.ci.size() + 5));
+                }
+                a.rp.push_back((int)a.ci.size());
+            }
+    return a;
+}
+
 // Random rows of 0..maxlen sorted distinct columns (some empty).
 static Csr random_rows(int nr, int nc, int maxlen, uint64_t seed) {
     Csr a;
@@ -278,6 +298,31 @@ int main() {
             char name[96];
             std::snprintf(name, sizeof name, "grid %dx%dx%d w%d h%d z%d", q.nx, q.ny, q.nz, q.w, q.h,
                           sh.zchunk);
+            rc |= check(name, a, sh, 8);
+        }
+    }
+    {
+        // 7-point (3D) and 5-point (2D) stencils: fewer shared columns per
+        // tile than the 27-point one, the same two row distances (a 2D grid
+        // has one: its tiles are consecutive rows walked along it)
+        struct { int nx, ny, nz, w, h; } g[] = {{20, 20, 20, 4, 8}, {20, 20, 20, 8, 4}, {13, 7, 9, 8, 8},
+                                               {120, 120, 1, 8, 4}, {33, 50, 1, 4, 8}};
+        for (const auto &q : g) {
+            Csr a = stencil7(q.nx, q.ny, q.nz);
+            RingShape sh;
+            sh.w = q.w;
+            sh.h = q.h;
+            sh.s_mid = q.nx;
+            sh.s_big = q.nz > 1 ? q.nx * q.ny : q.nx;
+            if (sh.s_big > 1 && (sh.s_mid < sh.w || sh.s_mid == sh.s_big)) {
+                sh.s_mid = sh.s_big;         // one distance, as the library shapes it
+                sh.w = q.w * q.h;
+                sh.h = 1;
+                if (sh.w > sh.s_big) sh.s_big = 1;
+            }
+            sh.zchunk = dspmm::ring_auto_chunk(a.nr, sh, 256);
+            char name[96];
+            std::snprintf(name, sizeof name, "7-point %dx%dx%d w%d h%d", q.nx, q.ny, q.nz, q.w, q.h);
             rc |= check(name, a, sh, 8);
         }
     }

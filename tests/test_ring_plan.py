@@ -4,7 +4,8 @@ simulated ring, checks that each nonzero's slot holds its column when its
 tile runs, that rows appear once and the kernel's limits hold, and that C
 evaluated through the plan equals the host loop bit for bit -- on grid
 stencils (ragged tile shapes, one and two row distances, a fixed larger
-ring, grids smaller than one tile, non-cubic grids), random rows in
+ring, grids smaller than one tile, non-cubic grids, 7-point and 5-point
+stencils), random rows in
 consecutive tiles (one row, a partial tile, far more columns than rows, a
 dozen columns shared by every row, mostly empty rows), random rows under
 arbitrary row-distance pairs (the library samples them, so an unstructured
@@ -32,4 +33,4 @@ def test_ring_plan_replays_exactly(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = [l for l in out.stdout.splitlines() if ": ok," in l]
-    assert len(lines) == 62, out.stdout
+    assert len(lines) == 67, out.stdout
