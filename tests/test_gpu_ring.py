@@ -243,6 +243,24 @@ def grid3(nx, ny, nz, seed=9):
     return Case(f"grid{nx}x{ny}x{nz}", n, n, rp, np.asarray(cols, np.int32), v, 64, seed=seed)
 
 
+def grid7(nx, ny, nz, seed=10):
+    """7-point stencil on an nx x ny x nz grid (nz = 1: the 5-point one)."""
+    rows, cols = [], []
+    offs = ((0, 0, -1), (0, -1, 0), (-1, 0, 0), (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                for dx, dy, dz in offs:
+                    xx, yy, zz = x + dx, y + dy, z + dz
+                    if 0 <= xx < nx and 0 <= yy < ny and 0 <= zz < nz:
+                        rows.append(x + nx * (y + ny * z))
+                        cols.append(xx + nx * (yy + ny * zz))
+    n = nx * ny * nz
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    v = np.random.default_rng(seed).uniform(-1, 1, len(cols)).astype(np.float32)
+    return Case(f"grid7pt_{nx}x{ny}x{nz}", n, n, rp, np.asarray(cols, np.int32), v, 64, seed=seed)
+
+
 def random_rows(name, nr, nc, maxlen, seed, keep_every=1):
     """Rows of 0..maxlen sorted distinct random columns; with keep_every k,
     all but one row in k are empty."""
@@ -265,6 +283,9 @@ EDGE = {
     "grid7x9x11": lambda: grid3(7, 9, 11),        # partial tiles on every edge
     "grid13x5x3": lambda: grid3(13, 5, 3),
     "grid37x2x2": lambda: grid3(37, 2, 2),
+    "grid7pt_20x20x20": lambda: grid7(20, 20, 20),   # 7-point stencil
+    "grid7pt_13x7x9": lambda: grid7(13, 7, 9),
+    "grid7pt_120x120x1": lambda: grid7(120, 120, 1),  # 5-point stencil: one row distance
     "one_row": lambda: random_rows("one_row", 1, 50, 20, 3),
     "nine_rows": lambda: random_rows("nine_rows", 9, 9, 9, 5),
     "wide": lambda: random_rows("wide", 4000, 100000, 30, 7),     # nc = 25 nr
@@ -275,8 +296,9 @@ EDGE = {
 
 @pytest.mark.parametrize("name", sorted(EDGE))
 def test_ring_edge_shapes(name):
-    """Tiny, ragged and lopsided matrices through the ring: bit for bit the
-    host loop at every tile shape, with FP32 and BF16-stored B.  A shape
+    """Tiny, ragged and lopsided matrices, and 7-point and 5-point stencils,
+    through the ring: bit for bit the host loop at every tile shape, with
+    FP32 and BF16-stored B.  A shape
     whose tile needs more B rows than LDS holds is refused; grids never
     are."""
     c = EDGE[name]()
