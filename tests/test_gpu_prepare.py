@@ -53,6 +53,9 @@ CASES = [
     ("zipf_mid", 1, RM, D.make_config(D.VARIANT_SPMV_TILED, tile_nnz=512)),
     ("edge", 16, CM, D.make_config(D.VARIANT_NNZ_TILED, tile_nnz=32)),         # transposed copies
     ("int_zipf", 8, RM, D.make_config(D.VARIANT_COO_VECTOR)),                  # zeroes C, rowIdx
+    # the B-row ring: its plan (and walk length) built by prepare
+    ("stencil_20", 64, RM, D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8)),
+    ("stencil_20", 32, CM, D.make_config(D.VARIANT_RING, tile_nnz=8, unroll=4, row_stride=10)),
 ]
 
 

@@ -2,7 +2,7 @@
 nothing (its scratch is a static device array) gives the same bits as
 dspmm_spmm_ex with the same configuration, on two calls in a row -- the
 second with a smaller input -- for split rows (carry scratch), column-major
-operands (transposed copies in the scratch) and AUTO; it refuses an
+operands (transposed copies in the scratch), AUTO and the B-row ring; it refuses an
 unprepared handle and a call whose scratch would exceed the maximum."""
 import ctypes as C
 
@@ -85,6 +85,25 @@ def test_noalloc_equals_spmm_ex_twice_second_smaller(layout, cfg):
     got = big.run_noalloc(layout, cfg)
     assert np.array_equal(bits(got), bits(want[id(big)]))
     note(f"no-alloc {layout} {cfg.as_dict() if cfg else 'auto'}: equal to spmm_ex, 3 calls")
+    big.close()
+    small.close()
+
+
+def test_noalloc_ring_on_stencils():
+    """The B-row ring needs no scratch: prepared, it runs through the
+    no-alloc entry point with dspmm_spmm_ex's bits (27-point stencils, the
+    second call smaller)."""
+    cfg = D.make_config(D.VARIANT_RING, tile_nnz=4, unroll=8)
+    big = Op(W.c3_matrix(20), 64, 45)
+    small = Op(W.c3_matrix(12), 32, 46)
+    for op in (big, small):
+        op.m.prepare(op.d_B, op.d_C, op.nv, config=cfg)
+    D.synchronize()
+    want = {id(op): op.reference(RM, cfg) for op in (big, small)}
+    for op in (big, small, big):
+        got = op.run_noalloc(RM, cfg)
+        assert np.array_equal(bits(got), bits(want[id(op)])), f"ring {op.a.nr} rows nv={op.nv}"
+    note("no-alloc ring: equal to spmm_ex, 3 calls")
     big.close()
     small.close()
 
