@@ -227,19 +227,23 @@ def test_server_bf16_operands():
     srv.close()
 
 
-def test_server_serves_a_stencil_on_its_ring():
+@pytest.mark.parametrize("bf16", [False, True], ids=["fp32", "bf16"])
+def test_server_serves_a_stencil_on_its_ring(bf16):
     """A 36^3 27-point stencil beside random matrices: its own AUTO kernel is
     the B-row ring (plan built at admission), requests of it alone or mixed
-    equal the host loop bit for bit on the device-generated B, and in a
-    burst they share batches with the other matrices."""
+    equal the host loop bit for bit on the device-generated B (with BF16-
+    stored B: on B rounded to nearest even), and in a burst they share
+    batches with the other matrices."""
+    btype = D.DSPMM_R_16BF if bf16 else D.DSPMM_R_32F
     pool = make_pool(n=4, seed=61)
     pool.append((W.c3_matrix(36), 64))
-    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16)
+    srv = D.Server(cache_bytes=1 << 30, operand_bytes=1 << 30, max_inflight=3, max_batch=16,
+                   btype=btype)
     mids = register(srv, pool)
     si = len(pool) - 1
     a, nv = pool[si]
     m = D.Matrix(a.nr, a.nc, a.rowPtr, a.colIdx, a.values)
-    assert m.resolve(nv).variant == D.VARIANT_RING
+    assert m.resolve(nv, btype=btype).variant == D.VARIANT_RING
     m.close()
     reqs, at, mi_of = [], [], {}
     rng = np.random.default_rng(4)
@@ -254,6 +258,8 @@ def test_server_serves_a_stencil_on_its_ring():
     got = results(srv, comps, pool, mi_of)
     for k in range(0, 40, 4):
         B = D.seeded_uniform(900 + k, a.nc * nv)
+        if bf16:
+            B = O.round_bf16(B)
         h32 = O.spmm_host32(RM, a.nr, a.nc, nv, a.rowPtr, a.colIdx, a.values, B).reshape(-1)
         assert np.array_equal(bits(got[k]), bits(h32)), f"stencil request {k}"
     note(f"stencil in the mix: stats {srv.stats()}")
