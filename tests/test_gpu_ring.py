@@ -213,6 +213,14 @@ def test_ring_integer_exact_and_column_major():
     D.synchronize()
     cm = s.output(32).reshape(32, s.nr).T.reshape(-1)
     assert np.array_equal(bits(cm), bits(rm)), "column-major ring differs from row-major"
+    # the same with BF16-stored B (its transposed copy is BF16 too)
+    rm16 = run(s, ms, 32, cfg, btype=D.DSPMM_R_16BF, B=D.to_bf16_device(s.d_B)).copy()
+    d_Bcm16 = D.to_bf16_device(d_Bcm)
+    s.fill_output(np.nan)
+    ms.spmm(d_Bcm16, s.d_C, 32, layout=CM, config=cfg, btype=D.DSPMM_R_16BF)
+    D.synchronize()
+    cm16 = s.output(32).reshape(32, s.nr).T.reshape(-1)
+    assert np.array_equal(bits(cm16), bits(rm16)), "column-major BF16 ring differs from row-major"
 
 
 def test_ring_runs_unstructured_small_tiles():
