@@ -24,6 +24,7 @@
 
 #include "batch_schedule.hpp"
 #include "common.hpp"
+#include "first_fit.hpp"
 #include "internal.hpp"
 #include "matrix.hpp"
 #include "native.hpp"
@@ -47,59 +48,30 @@ constexpr int64_t kServeAloneNnz = 16 << 20;
 // kernel, and launch one after another on the slot's stream.
 constexpr size_t kServeOwnMax = 4;
 
-// First-fit allocator over one device region; offsets aligned to 256 B.
+// The operand arena: one device region, its blocks handed out first-fit
+// (offsets aligned to 256 B).
 class Arena {
   public:
     dspmm_status_t init(size_t bytes) {
-        size_ = round_up(bytes, 256);
-        if (size_ == 0) return DSPMM_STATUS_SUCCESS;
-        dspmm_status_t s = hip_status(hipMalloc((void **)&base_, size_));
-        if (s == DSPMM_STATUS_SUCCESS) free_[0] = size_;
+        const size_t size = round_up(bytes, 256);
+        if (size == 0) return DSPMM_STATUS_SUCCESS;
+        dspmm_status_t s = hip_status(hipMalloc((void **)&base_, size));
+        if (s == DSPMM_STATUS_SUCCESS) ff_.reset(size);
         return s;
     }
     void destroy() {
         if (base_) (void)hipFree(base_);
         base_ = nullptr;
     }
-    bool alloc(size_t n, size_t *off) {
-        n = round_up(std::max<size_t>(n, 1), 256);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < n) continue;
-            *off = it->first;
-            const size_t rest = it->second - n, start = it->first + n;
-            free_.erase(it);
-            if (rest) free_[start] = rest;
-            used_ += n;
-            peak_ = std::max(peak_, used_);
-            return true;
-        }
-        return false;
-    }
-    void release(size_t off, size_t n) {
-        n = round_up(std::max<size_t>(n, 1), 256);
-        used_ -= n;
-        auto it = free_.emplace(off, n).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
+    bool alloc(size_t n, size_t *off) { return ff_.alloc(n, off); }
+    void release(size_t off, size_t n) { ff_.release(off, n); }
     char *ptr(size_t off) const { return base_ + off; }
-    size_t size() const { return size_; }
-    size_t peak() const { return peak_; }
+    size_t size() const { return ff_.size(); }
+    size_t peak() const { return ff_.peak(); }
 
   private:
     char *base_ = nullptr;
-    size_t size_ = 0, used_ = 0, peak_ = 0;
-    std::map<size_t, size_t> free_;
+    FirstFit ff_;
 };
 
 // Per (matrix, nv): the matrix's part of a batch schedule, and whether a
